@@ -1,0 +1,241 @@
+"""CPU oracle for the voxurf_amd HIP library -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg may import this
+package, and only as the checker.  No product code path calls it.
+
+  * oracle.c (built to oracle/_build/liboracle.so): sequential C restatement of the
+    reference's native ops, float and double, contraction off.  Wrapped here with numpy.
+  * oracle/torch_ref.py: restatement of the Python-level hot path (grid_sample based
+    trilinear gather, 6-neighbour normals, NeuS alpha, segment sums, MLP, Adam, the
+    coarse and fine training steps) in PyTorch on the CPU, float32 or float64.
+
+The reference itself (CUDA sources compiled by torch's JIT at import time) cannot be
+built in this environment, so there is no oracle/_ref build.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+SRC = os.path.join(HERE, "oracle.c")
+
+CFLAGS = ["-O2", "-std=c11", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-shared", "-Wall"]
+
+
+def build():
+    """Compile oracle.c with gcc (no FMA contraction, SSE2 float arithmetic)."""
+    os.makedirs(os.path.dirname(LIB_PATH), exist_ok=True)
+    tmp = LIB_PATH + ".tmp"
+    subprocess.run(["gcc", *CFLAGS, SRC, "-o", tmp, "-lm"], check=True)
+    os.replace(tmp, LIB_PATH)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < os.path.getmtime(SRC):
+            build()
+        _lib = ctypes.CDLL(LIB_PATH)
+    return _lib
+
+
+def _sfx(a):
+    if a.dtype == np.float32:
+        return "f32"
+    if a.dtype == np.float64:
+        return "f64"
+    raise TypeError(f"unsupported dtype {a.dtype}")
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _c(a, dtype=None):
+    a = np.ascontiguousarray(a if dtype is None else np.asarray(a, dtype=dtype))
+    return a
+
+
+_F = ctypes.c_float
+_I64 = ctypes.c_int64
+_I = ctypes.c_int
+
+
+def _fn(name, restype=None):
+    f = getattr(lib(), name)
+    f.restype = restype
+    return f
+
+
+# ---- sampling (render_utils_kernel.cu:12-242) ---------------------------------------
+
+def infer_t_minmax(rays_o, rays_d, xyz_min, xyz_max, near, far):
+    rays_o = _c(rays_o)
+    T = rays_o.dtype
+    rays_d, xyz_min, xyz_max = _c(rays_d, T), _c(xyz_min, T), _c(xyz_max, T)
+    n = rays_o.shape[0]
+    t_min = np.empty(n, T)
+    t_max = np.empty(n, T)
+    _fn(f"or_infer_t_minmax_{_sfx(rays_o)}")(_p(rays_o), _p(rays_d), _p(xyz_min), _p(xyz_max),
+                                              _F(near), _F(far), _I64(n), _p(t_min), _p(t_max))
+    return t_min, t_max
+
+
+def infer_n_samples(rays_d, t_min, t_max, stepdist):
+    t_min = _c(t_min)
+    T = t_min.dtype
+    rays_d, t_max = _c(rays_d, T), _c(t_max, T)
+    n = t_min.shape[0]
+    out = np.empty(n, np.int64)
+    _fn(f"or_infer_n_samples_{_sfx(t_min)}")(_p(rays_d), _p(t_min), _p(t_max), _F(stepdist),
+                                              _I64(n), _p(out))
+    return out
+
+
+def infer_ray_start_dir(rays_o, rays_d, t_min):
+    rays_o = _c(rays_o)
+    T = rays_o.dtype
+    rays_d, t_min = _c(rays_d, T), _c(t_min, T)
+    n = rays_o.shape[0]
+    s = np.empty((n, 3), T)
+    d = np.empty((n, 3), T)
+    _fn(f"or_infer_ray_start_dir_{_sfx(rays_o)}")(_p(rays_o), _p(rays_d), _p(t_min), _I64(n),
+                                                   _p(s), _p(d))
+    return s, d
+
+
+def sample_pts_on_rays(rays_o, rays_d, xyz_min, xyz_max, near, far, stepdist):
+    """-> (rays_pts, mask_outbbox, ray_id, step_id, N_steps, t_min, t_max)"""
+    rays_o = _c(rays_o)
+    T = rays_o.dtype
+    rays_d, xyz_min, xyz_max = _c(rays_d, T), _c(xyz_min, T), _c(xyz_max, T)
+    n = rays_o.shape[0]
+    sfx = _sfx(rays_o)
+    t_min = np.empty(n, T)
+    t_max = np.empty(n, T)
+    n_steps = np.empty(n, np.int64)
+    total = _fn(f"or_sample_count_{sfx}", ctypes.c_int64)(
+        _p(rays_o), _p(rays_d), _p(xyz_min), _p(xyz_max), _F(near), _F(far), _F(stepdist),
+        _I64(n), _p(t_min), _p(t_max), _p(n_steps))
+    pts = np.empty((total, 3), T)
+    mask = np.empty(total, np.uint8)
+    ray_id = np.empty(total, np.int64)
+    step_id = np.empty(total, np.int64)
+    _fn(f"or_sample_fill_{sfx}")(_p(rays_o), _p(rays_d), _p(xyz_min), _p(xyz_max), _p(t_min),
+                                  _F(stepdist), _I64(n), _p(n_steps), _I64(total), _p(pts),
+                                  _p(mask), _p(ray_id), _p(step_id))
+    return pts, mask.astype(bool), ray_id, step_id, n_steps, t_min, t_max
+
+
+def sample_ndc_pts_on_rays(rays_o, rays_d, xyz_min, xyz_max, N_samples):
+    rays_o = _c(rays_o)
+    T = rays_o.dtype
+    rays_d, xyz_min, xyz_max = _c(rays_d, T), _c(xyz_min, T), _c(xyz_max, T)
+    n = rays_o.shape[0]
+    pts = np.empty((n, N_samples, 3), T)
+    mask = np.empty((n, N_samples), np.uint8)
+    _fn(f"or_sample_ndc_{_sfx(rays_o)}")(_p(rays_o), _p(rays_d), _p(xyz_min), _p(xyz_max),
+                                          _I(N_samples), _I64(n), _p(pts), _p(mask))
+    return pts, mask.astype(bool)
+
+
+def sample_bg_pts_on_rays(rays_o, rays_d, t_max, bg_preserve, N_samples):
+    rays_o = _c(rays_o)
+    T = rays_o.dtype
+    rays_d, t_max = _c(rays_d, T), _c(t_max, T)
+    n = rays_o.shape[0]
+    pts = np.empty((n, N_samples, 3), T)
+    _fn(f"or_sample_bg_{_sfx(rays_o)}")(_p(rays_o), _p(rays_d), _p(t_max), _F(bg_preserve),
+                                         _I(N_samples), _I64(n), _p(pts))
+    return pts
+
+
+def maskcache_lookup(world, xyz, scale, shift):
+    xyz = _c(xyz)
+    T = xyz.dtype
+    world = _c(world, np.uint8)
+    scale, shift = _c(scale, T), _c(shift, T)
+    n = xyz.shape[0]
+    out = np.empty(n, np.uint8)
+    si, sj, sk = world.shape
+    _fn(f"or_maskcache_lookup_{_sfx(xyz)}")(_p(world), _p(xyz), _p(scale), _p(shift), _I(si),
+                                             _I(sj), _I(sk), _I64(n), _p(out))
+    return out.astype(bool)
+
+
+# ---- raw2alpha / alpha2weight (render_utils_kernel.cu:431-707) -----------------------
+
+def raw2alpha(density, shift, interval):
+    density = _c(density)
+    n = density.shape[0]
+    e = np.empty_like(density)
+    a = np.empty_like(density)
+    _fn(f"or_raw2alpha_{_sfx(density)}")(_p(density), _F(shift), _F(interval), None, _I64(n),
+                                          _p(e), _p(a))
+    return e, a
+
+
+def raw2alpha_nonuni(density, shift, interval):
+    density = _c(density)
+    interval = _c(interval, density.dtype)
+    n = density.shape[0]
+    e = np.empty_like(density)
+    a = np.empty_like(density)
+    _fn(f"or_raw2alpha_{_sfx(density)}")(_p(density), _F(shift), _F(0.0), _p(interval), _I64(n),
+                                          _p(e), _p(a))
+    return e, a
+
+
+def raw2alpha_backward(exp_d, grad_back, interval):
+    exp_d = _c(exp_d)
+    grad_back = _c(grad_back, exp_d.dtype)
+    g = np.empty_like(exp_d)
+    _fn(f"or_raw2alpha_backward_{_sfx(exp_d)}")(_p(exp_d), _p(grad_back), _F(interval), None,
+                                                 _I64(exp_d.shape[0]), _p(g))
+    return g
+
+
+def raw2alpha_nonuni_backward(exp_d, grad_back, interval):
+    exp_d = _c(exp_d)
+    grad_back = _c(grad_back, exp_d.dtype)
+    interval = _c(interval, exp_d.dtype)
+    g = np.empty_like(exp_d)
+    _fn(f"or_raw2alpha_backward_{_sfx(exp_d)}")(_p(exp_d), _p(grad_back), _F(0.0), _p(interval),
+                                                 _I64(exp_d.shape[0]), _p(g))
+    return g
+
+
+def alpha2weight(alpha, ray_id, n_rays):
+    """-> (weight, T, alphainv_last, i_start, i_end)"""
+    alpha = _c(alpha)
+    ray_id = _c(ray_id, np.int64)
+    n = alpha.shape[0]
+    w = np.empty_like(alpha)
+    T = np.empty_like(alpha)
+    last = np.empty(n_rays, alpha.dtype)
+    i_s = np.empty(n_rays, np.int64)
+    i_e = np.empty(n_rays, np.int64)
+    _fn(f"or_alpha2weight_{_sfx(alpha)}")(_p(alpha), _p(ray_id), _I64(n), _I64(n_rays), _p(w),
+                                           _p(T), _p(last), _p(i_s), _p(i_e))
+    return w, T, last, i_s, i_e
+
+
+def alpha2weight_backward(alpha, weight, T, alphainv_last, i_start, i_end, n_rays,
+                          grad_weights, grad_last):
+    alpha = _c(alpha)
+    D = alpha.dtype
+    weight, T, alphainv_last = _c(weight, D), _c(T, D), _c(alphainv_last, D)
+    grad_weights, grad_last = _c(grad_weights, D), _c(grad_last, D)
+    i_start, i_end = _c(i_start, np.int64), _c(i_end, np.int64)
+    g = np.empty_like(alpha)
+    _fn(f"or_alpha2weight_backward_{_sfx(alpha)}")(
+        _p(alpha), _p(weight), _p(T), _p(alphainv_last), _p(i_start), _p(i_end),
+        _I64(alpha.shape[0]), _I64(n_rays), _p(grad_weights), _p(grad_last), _p(g))
+    return g

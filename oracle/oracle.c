@@ -1,0 +1,284 @@
+/*
+ * oracle.c -- CPU restatement of Voxurf's native ray-marching ops.  TEST
+ * INFRASTRUCTURE ONLY: it is the checker for the HIP library (tests/, the smoke check
+ * and the CPU-baseline leg of bench.py load it); no product code path calls it.
+ *
+ * Each function follows one kernel of the reference (paths relative to the Voxurf
+ * repository) statement by statement and sequentially, one ray or one element at a
+ * time, with the reference's own mix of float and double temporaries.  It is built
+ * with gcc -ffp-contract=off (SSE2 arithmetic, no fused multiply-add), which is the
+ * contraction policy the HIP kernels use for everything that feeds an integer output.
+ *
+ * Every function exists for float (_f32) and double (_f64) tensors, mirroring
+ * AT_DISPATCH_FLOATING_TYPES.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+static inline float fminf_(float a, float b) { return fminf(a, b); }
+static inline float fmaxf_(float a, float b) { return fmaxf(a, b); }
+
+/* float -> int32 as a GPU cvt.rzi.s32.f32: NaN -> 0, saturating at the int32 range. */
+static inline int sat_f2i_d(double x) {
+  if (x != x) return 0;
+  if (x >= 2147483648.0) return 2147483647;
+  if (x <= -2147483648.0) return -2147483647 - 1;
+  return (int)x;
+}
+
+#define DEFINE_ORACLE(T, SFX)                                                                 \
+  /* render_utils_kernel.cu:12-35 */                                                          \
+  static void t_minmax_##SFX(const T* o, const T* d, const T* mn, const T* mx, float near_,   \
+                             float far_, float* tmin, float* tmax) {                         \
+    const float vx = (d[0] == 0) ? (float)1e-6 : (float)d[0];                                 \
+    const float vy = (d[1] == 0) ? (float)1e-6 : (float)d[1];                                 \
+    const float vz = (d[2] == 0) ? (float)1e-6 : (float)d[2];                                 \
+    const float ax = (float)((mx[0] - o[0]) / (T)vx);                                         \
+    const float ay = (float)((mx[1] - o[1]) / (T)vy);                                         \
+    const float az = (float)((mx[2] - o[2]) / (T)vz);                                         \
+    const float bx = (float)((mn[0] - o[0]) / (T)vx);                                         \
+    const float by = (float)((mn[1] - o[1]) / (T)vy);                                         \
+    const float bz = (float)((mn[2] - o[2]) / (T)vz);                                         \
+    *tmin = fmaxf_(fminf_(fmaxf_(fmaxf_(fminf_(ax, bx), fminf_(ay, by)), fminf_(az, bz)),     \
+                          far_), near_);                                                     \
+    *tmax = fmaxf_(fminf_(fminf_(fminf_(fmaxf_(ax, bx), fmaxf_(ay, by)), fmaxf_(az, bz)),     \
+                          far_), near_);                                                     \
+  }                                                                                           \
+  static float rnorm_##SFX(const T* d) {                                                      \
+    const T s = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];                                      \
+    return (float)SQRT_##SFX(s);                                                              \
+  }                                                                                           \
+  void or_infer_t_minmax_##SFX(const T* rays_o, const T* rays_d, const T* mn, const T* mx,    \
+                               float near_, float far_, int64_t n, T* t_min, T* t_max) {     \
+    for (int64_t r = 0; r < n; ++r) {                                                         \
+      float a, b;                                                                             \
+      t_minmax_##SFX(rays_o + 3 * r, rays_d + 3 * r, mn, mx, near_, far_, &a, &b);            \
+      t_min[r] = (T)a;                                                                        \
+      t_max[r] = (T)b;                                                                        \
+    }                                                                                         \
+  }                                                                                           \
+  /* render_utils_kernel.cu:38-55 */                                                          \
+  void or_infer_n_samples_##SFX(const T* rays_d, const T* t_min, const T* t_max,              \
+                                float stepdist, int64_t n, int64_t* ns) {                     \
+    for (int64_t r = 0; r < n; ++r) {                                                         \
+      const float rn = rnorm_##SFX(rays_d + 3 * r);                                           \
+      const T q = (t_max[r] - t_min[r]) * (T)rn / (T)stepdist;                                \
+      const double c = ceil((double)q);                                                       \
+      ns[r] = (int64_t)(c > 1.0 ? c : 1.0);                                                   \
+    }                                                                                         \
+  }                                                                                           \
+  /* render_utils_kernel.cu:58-79 */                                                          \
+  void or_infer_ray_start_dir_##SFX(const T* rays_o, const T* rays_d, const T* t_min,         \
+                                    int64_t n, T* start, T* dir) {                            \
+    for (int64_t r = 0; r < n; ++r) {                                                         \
+      const T* o = rays_o + 3 * r;                                                            \
+      const T* d = rays_d + 3 * r;                                                            \
+      const float rn = rnorm_##SFX(d);                                                        \
+      for (int c = 0; c < 3; ++c) {                                                           \
+        start[3 * r + c] = o[c] + d[c] * t_min[r];                                            \
+        dir[3 * r + c] = d[c] / (T)rn;                                                        \
+      }                                                                                       \
+    }                                                                                         \
+  }                                                                                           \
+  /* render_utils_kernel.cu:196-242, in two calls: count (N_steps and its total), then     \
+     fill.  The fill walks the flattened sample index exactly as the reference's          \
+     __set_1_at_ray_seg_start + cumsum + __set_step_id + sampling kernel do. */              \
+  int64_t or_sample_count_##SFX(const T* rays_o, const T* rays_d, const T* mn, const T* mx,   \
+                                float near_, float far_, float stepdist, int64_t n,           \
+                                T* t_min, T* t_max, int64_t* n_steps) {                       \
+    or_infer_t_minmax_##SFX(rays_o, rays_d, mn, mx, near_, far_, n, t_min, t_max);            \
+    or_infer_n_samples_##SFX(rays_d, t_min, t_max, stepdist, n, n_steps);                     \
+    int64_t tot = 0;                                                                          \
+    for (int64_t r = 0; r < n; ++r) tot += n_steps[r];                                        \
+    return tot;                                                                               \
+  }                                                                                           \
+  void or_sample_fill_##SFX(const T* rays_o, const T* rays_d, const T* mn, const T* mx,       \
+                            const T* t_min, float stepdist, int64_t n, const int64_t* n_steps,\
+                            int64_t total, T* pts, uint8_t* mask, int64_t* ray_id,            \
+                            int64_t* step_id) {                                               \
+    /* __set_1_at_ray_seg_start: ray_id[cumsum[r-1]] = 1 for r > 0, then cumsum_. */        \
+    int64_t acc = 0;                                                                          \
+    for (int64_t m = 0; m < total; ++m) ray_id[m] = 0;                                        \
+    for (int64_t r = 0; r < n; ++r) {                                                         \
+      if (r > 0 && acc < total) ray_id[acc] = 1;                                              \
+      acc += n_steps[r];                                                                      \
+    }                                                                                         \
+    for (int64_t m = 1; m < total; ++m) ray_id[m] += ray_id[m - 1];                           \
+    /* __set_step_id: step_id[m] = m - cumsum[ray_id[m]-1]. */                                \
+    {                                                                                         \
+      int64_t r = 0, seg_beg = 0;                                                             \
+      for (int64_t m = 0; m < total; ++m) {                                                   \
+        while (r < ray_id[m]) seg_beg += n_steps[r++];                                        \
+        step_id[m] = m - seg_beg;                                                             \
+      }                                                                                       \
+    }                                                                                         \
+    for (int64_t m = 0; m < total; ++m) {                                                     \
+      const int64_t r = ray_id[m];                                                            \
+      const int i_step = (int)step_id[m];                                                     \
+      const T* o = rays_o + 3 * r;                                                            \
+      const T* d = rays_d + 3 * r;                                                            \
+      const float rn = rnorm_##SFX(d);                                                        \
+      T st[3], dr[3];                                                                         \
+      for (int c = 0; c < 3; ++c) {                                                           \
+        st[c] = o[c] + d[c] * t_min[r];                                                       \
+        dr[c] = d[c] / (T)rn;                                                                 \
+      }                                                                                       \
+      const float dist = stepdist * (float)i_step;                                            \
+      const float px = (float)(st[0] + dr[0] * (T)dist);                                      \
+      const float py = (float)(st[1] + dr[1] * (T)dist);                                      \
+      const float pz = (float)(st[2] + dr[2] * (T)dist);                                      \
+      pts[3 * m + 0] = (T)px;                                                                 \
+      pts[3 * m + 1] = (T)py;                                                                 \
+      pts[3 * m + 2] = (T)pz;                                                                 \
+      mask[m] = (mn[0] > (T)px) | (mn[1] > (T)py) | (mn[2] > (T)pz) | (mx[0] < (T)px) |       \
+                (mx[1] < (T)py) | (mx[2] < (T)pz);                                            \
+    }                                                                                         \
+  }                                                                                           \
+  /* render_utils_kernel.cu:244-270 */                                                        \
+  void or_sample_ndc_##SFX(const T* rays_o, const T* rays_d, const T* mn, const T* mx,        \
+                           int ns, int64_t n, T* pts, uint8_t* mask) {                        \
+    for (int64_t idx = 0; idx < n * ns; ++idx) {                                              \
+      const int64_t r = idx / ns;                                                             \
+      const int s = (int)(idx % ns);                                                          \
+      const float dist = ((float)s) / (float)(ns - 1);                                        \
+      const T* o = rays_o + 3 * r;                                                            \
+      const T* d = rays_d + 3 * r;                                                            \
+      const float px = (float)(o[0] + d[0] * (T)dist);                                        \
+      const float py = (float)(o[1] + d[1] * (T)dist);                                        \
+      const float pz = (float)(o[2] + d[2] * (T)dist);                                        \
+      pts[3 * idx + 0] = (T)px;                                                               \
+      pts[3 * idx + 1] = (T)py;                                                               \
+      pts[3 * idx + 2] = (T)pz;                                                               \
+      mask[idx] = (mn[0] > (T)px) | (mn[1] > (T)py) | (mn[2] > (T)pz) | (mx[0] < (T)px) |     \
+                  (mx[1] < (T)py) | (mx[2] < (T)pz);                                          \
+    }                                                                                         \
+  }                                                                                           \
+  /* render_utils_kernel.cu:300-340 */                                                        \
+  void or_sample_bg_##SFX(const T* rays_o, const T* rays_d, const T* t_max, float bgp,        \
+                          int ns, int64_t n, T* pts) {                                        \
+    for (int64_t idx = 0; idx < n * ns; ++idx) {                                              \
+      const int64_t r = idx / ns;                                                             \
+      const int s = (int)(idx % ns);                                                          \
+      const T* o = rays_o + 3 * r;                                                            \
+      const T* d = rays_d + 3 * r;                                                            \
+      const float t_inner = (float)t_max[r];                                                  \
+      const float frac = ((float)s) / (float)ns;                                              \
+      const float t0 = (float)((double)t_inner - 1. + 1. / (1. - (double)frac));              \
+      const float x = (float)(o[0] + d[0] * (T)t0);                                           \
+      const float y = (float)(o[1] + d[1] * (T)t0);                                           \
+      const float z = (float)(o[2] + d[2] * (T)t0);                                           \
+      const float t_outer = sqrtf(x * x + y * y + z * z);                                     \
+      const float m = fmaxf(fabsf(x), fmaxf(fabsf(y), fabsf(z)));                             \
+      const float R = t_outer / m;                                                            \
+      const float o2i = (float)((double)(R * R / (t_outer * t_outer)) * (1. - (double)bgp) +  \
+                                (double)(R / t_outer * bgp));                                 \
+      pts[3 * idx + 0] = (T)(x * o2i);                                                        \
+      pts[3 * idx + 1] = (T)(y * o2i);                                                        \
+      pts[3 * idx + 2] = (T)(z * o2i);                                                        \
+    }                                                                                         \
+  }                                                                                           \
+  /* render_utils_kernel.cu:374-392 */                                                        \
+  void or_maskcache_lookup_##SFX(const uint8_t* world, const T* xyz, const T* scale,          \
+                                 const T* shift, int si, int sj, int sk, int64_t n,           \
+                                 uint8_t* out) {                                              \
+    for (int64_t p = 0; p < n; ++p) {                                                         \
+      const T* x = xyz + 3 * p;                                                               \
+      const int i = sat_f2i_d((double)ROUND_##SFX(x[0] * scale[0] + shift[0]));               \
+      const int j = sat_f2i_d((double)ROUND_##SFX(x[1] * scale[1] + shift[1]));               \
+      const int k = sat_f2i_d((double)ROUND_##SFX(x[2] * scale[2] + shift[2]));               \
+      uint8_t v = 0;                                                                          \
+      if (0 <= i && i < si && 0 <= j && j < sj && 0 <= k && k < sk)                           \
+        v = world[((int64_t)i * sj + j) * sk + k] != 0;                                       \
+      out[p] = v;                                                                             \
+    }                                                                                         \
+  }                                                                                           \
+  /* render_utils_kernel.cu:431-458 (interval_arr == NULL: uniform interval) */              \
+  void or_raw2alpha_##SFX(const T* density, float shift, float interval,                      \
+                          const T* interval_arr, int64_t n, T* exp_d, T* alpha) {             \
+    for (int64_t i = 0; i < n; ++i) {                                                         \
+      const T itv = interval_arr ? interval_arr[i] : (T)interval;                             \
+      const T e = EXP_##SFX(density[i] + (T)shift);                                           \
+      exp_d[i] = e;                                                                           \
+      alpha[i] = (T)1 - POW_##SFX((T)1 + e, -itv);                                            \
+    }                                                                                         \
+  }                                                                                           \
+  /* render_utils_kernel.cu:507-530 */                                                        \
+  void or_raw2alpha_backward_##SFX(const T* exp_d, const T* gb, float interval,               \
+                                   const T* interval_arr, int64_t n, T* grad) {               \
+    for (int64_t i = 0; i < n; ++i) {                                                         \
+      const T itv = interval_arr ? interval_arr[i] : (T)interval;                             \
+      const T e = exp_d[i];                                                                   \
+      const double ec = fmin((double)e, 1e10);                                                \
+      const T pw = POW_##SFX((T)1 + e, -itv - (T)1);                                          \
+      grad[i] = (T)(((ec * (double)pw) * (double)itv) * (double)gb[i]);                       \
+    }                                                                                         \
+  }                                                                                           \
+  /* render_utils_kernel.cu:577-651: the sequential per-ray transmittance walk. */           \
+  void or_alpha2weight_##SFX(const T* alpha, const int64_t* ray_id, int64_t n_pts,            \
+                             int64_t n_rays, T* weight, T* Tt, T* last, int64_t* i_start,     \
+                             int64_t* i_end) {                                                \
+    for (int64_t i = 0; i < n_pts; ++i) {                                                     \
+      weight[i] = 0;                                                                          \
+      Tt[i] = 1;                                                                              \
+    }                                                                                         \
+    for (int64_t r = 0; r < n_rays; ++r) {                                                    \
+      last[r] = 1;                                                                            \
+      i_start[r] = 0;                                                                         \
+      i_end[r] = 0;                                                                           \
+    }                                                                                         \
+    if (n_pts == 0) return;                                                                   \
+    for (int64_t i = 1; i < n_pts; ++i) {                                                     \
+      if (ray_id[i] != ray_id[i - 1]) {                                                       \
+        if (ray_id[i] >= 0 && ray_id[i] < n_rays) i_start[ray_id[i]] = i;                     \
+        if (ray_id[i - 1] >= 0 && ray_id[i - 1] < n_rays) i_end[ray_id[i - 1]] = i;           \
+      }                                                                                       \
+    }                                                                                         \
+    if (ray_id[n_pts - 1] >= 0 && ray_id[n_pts - 1] < n_rays) i_end[ray_id[n_pts - 1]] = n_pts;\
+    for (int64_t r = 0; r < n_rays; ++r) {                                                    \
+      const int64_t is = i_start[r];                                                          \
+      const int64_t iemax = i_end[r];                                                         \
+      float Tc = 1.f;                                                                         \
+      int64_t i;                                                                              \
+      for (i = is; i < iemax; ++i) {                                                          \
+        Tt[i] = (T)Tc;                                                                        \
+        weight[i] = (T)Tc * alpha[i];                                                         \
+        Tc = (float)((double)Tc * (1.0 - (double)alpha[i]));                                  \
+        if ((double)Tc < 1e-3) {                                                              \
+          i += 1;                                                                             \
+          break;                                                                              \
+        }                                                                                     \
+      }                                                                                       \
+      i_end[r] = i;                                                                           \
+      last[r] = (T)Tc;                                                                        \
+    }                                                                                         \
+  }                                                                                           \
+  /* render_utils_kernel.cu:654-707 (sequential float accumulator, as the reference). */     \
+  void or_alpha2weight_backward_##SFX(const T* alpha, const T* weight, const T* Tt,           \
+                                      const T* last, const int64_t* i_start,                  \
+                                      const int64_t* i_end, int64_t n_pts, int64_t n_rays,    \
+                                      const T* gw, const T* glast, T* grad) {                 \
+    for (int64_t i = 0; i < n_pts; ++i) grad[i] = 0;                                          \
+    for (int64_t r = 0; r < n_rays; ++r) {                                                    \
+      const int64_t is = i_start[r];                                                          \
+      const int64_t ie = i_end[r];                                                            \
+      float back = (float)(glast[r] * last[r]);                                               \
+      for (int64_t i = ie - 1; i >= is; --i) {                                                \
+        const double den = (double)((T)1 - alpha[i]) + 1e-10;                                 \
+        grad[i] = (T)((double)(gw[i] * Tt[i]) - (double)back / den);                          \
+        back = (float)((T)back + gw[i] * weight[i]);                                          \
+      }                                                                                       \
+    }                                                                                         \
+  }
+
+#define SQRT_f32(x) sqrtf(x)
+#define SQRT_f64(x) sqrt(x)
+#define ROUND_f32(x) roundf(x)
+#define ROUND_f64(x) round(x)
+#define EXP_f32(x) expf(x)
+#define EXP_f64(x) exp(x)
+#define POW_f32(a, b) powf(a, b)
+#define POW_f64(a, b) pow(a, b)
+
+DEFINE_ORACLE(float, f32)
+DEFINE_ORACLE(double, f64)

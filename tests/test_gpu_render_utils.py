@@ -1,0 +1,153 @@
+"""HIP render_utils ops vs the C oracle on identical inputs (GPU).
+
+Integer and index outputs must match bit for bit (N_steps, ray_id, step_id,
+mask_outbbox, i_start, i_end, mask lookups).  The sampling geometry (t_min, t_max,
+rays_pts) and the transmittance walk (weight, T, alphainv_last) use the same operation
+order and rounding as the oracle, so they are compared exactly as well.  Results of
+exp/pow (raw2alpha) and of the reordered suffix sums of alpha2weight_backward are
+compared within 1e-5 relative (forward) and 1e-4 relative (gradients).
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from tests.raysets import mixed_rays
+from voxurf_amd import render_utils_cuda as ru
+
+pytestmark = pytest.mark.gpu
+
+BOX = (np.array([-1.0, -1.0, -1.0]), np.array([1.0, 1.0, 1.0]))
+
+
+def _t(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def _eq(a, b):
+    a = a.detach().cpu().numpy() if isinstance(a, torch.Tensor) else a
+    np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("near,step", [(0.0, 0.5 * 2 / 256), (0.2, 0.5 * 2 / 96)])
+def test_sample_pts_on_rays_bitexact(dev, dtype, near, step):
+    o, d = mixed_rays(3000, seed=11, dtype=dtype)
+    mn, mx = BOX[0].astype(dtype), BOX[1].astype(dtype)
+    ref = oracle.sample_pts_on_rays(o, d, mn, mx, near, 1e9, np.float32(step))
+    out = ru.sample_pts_on_rays(_t(o, dev), _t(d, dev), _t(mn, dev), _t(mx, dev), near, 1e9,
+                                float(np.float32(step)))
+    names = ["rays_pts", "mask_outbbox", "ray_id", "step_id", "N_steps", "t_min", "t_max"]
+    for name, a, b in zip(names, out, ref):
+        print(name, tuple(a.shape))
+        _eq(a, b)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_infer_ops_bitexact(dev, dtype):
+    o, d = mixed_rays(1000, seed=12, dtype=dtype)
+    mn, mx = BOX[0].astype(dtype), BOX[1].astype(dtype)
+    tmin, tmax = oracle.infer_t_minmax(o, d, mn, mx, 0.0, 1e9)
+    a, b = ru.infer_t_minmax(_t(o, dev), _t(d, dev), _t(mn, dev), _t(mx, dev), 0.0, 1e9)
+    _eq(a, tmin)
+    _eq(b, tmax)
+    ns = ru.infer_n_samples(_t(d, dev), a, b, 0.01)
+    _eq(ns, oracle.infer_n_samples(d, tmin, tmax, 0.01))
+    s, dd = ru.infer_ray_start_dir(_t(o, dev), _t(d, dev), a)
+    rs, rd = oracle.infer_ray_start_dir(o, d, tmin)
+    _eq(s, rs)
+    _eq(dd, rd)
+
+
+def test_sample_pts_large_batch_properties(dev):
+    """65536 scene rays at 256^3 spacing: exact vs the oracle, plus layout invariants."""
+    from voxurf_amd import scene
+    o, d, _ = scene.rays(65536, seed=3)
+    step = scene.FINE.stepdist
+    mn, mx = torch.tensor(BOX[0], dtype=torch.float32), torch.tensor(BOX[1], dtype=torch.float32)
+    out = ru.sample_pts_on_rays(o.to(dev), d.to(dev), mn.to(dev), mx.to(dev), 0.0, 1e9, step)
+    pts, mask, rid, sid, ns = [t.cpu() for t in out[:5]]
+    assert int(ns.sum()) == len(rid)
+    assert bool((rid[1:] >= rid[:-1]).all())
+    assert torch.equal(torch.bincount(rid, minlength=65536), ns)
+    ref = oracle.sample_pts_on_rays(o.numpy(), d.numpy(), BOX[0].astype(np.float32),
+                                    BOX[1].astype(np.float32), 0.0, 1e9, np.float32(step))
+    for a, b in zip((pts, mask, rid, sid, ns), ref):
+        _eq(a, b)
+
+
+def test_empty_batches(dev):
+    z = torch.zeros((0, 3), device=dev)
+    mn = torch.full((3,), -1.0, device=dev)
+    out = ru.sample_pts_on_rays(z, z, mn, -mn, 0.0, 1e9, 0.01)
+    assert all(t.shape[0] == 0 for t in out)
+    w, T, last, i_s, i_e = ru.alpha2weight(torch.zeros(0, device=dev),
+                                           torch.zeros(0, dtype=torch.int64, device=dev), 5)
+    _eq(last, np.ones(5, np.float32))
+    _eq(i_e, np.zeros(5, np.int64))
+    e, a = ru.raw2alpha(torch.zeros(0, device=dev), 0.0, 1.0)
+    assert a.numel() == 0
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_maskcache_bitexact(dev, dtype):
+    rng = np.random.default_rng(13)
+    world = rng.random((20, 17, 9)) > 0.4
+    xyz = rng.uniform(-1.2, 1.2, (50000, 3)).astype(dtype)
+    xyz[:4] = [[0.5, 0.5, 0.5], [np.nan, 0, 0], [1e12, 0, 0], [-0.5, -0.5, -0.5]]
+    sc = np.array([9.5, 8.0, 4.0], dtype)
+    sh = np.array([9.5, 8.0, 4.0], dtype)
+    ref = oracle.maskcache_lookup(world, xyz, sc, sh)
+    out = ru.maskcache_lookup(_t(world, dev), _t(xyz, dev), _t(sc, dev), _t(sh, dev))
+    _eq(out, ref)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_raw2alpha_family(dev, dtype):
+    rng = np.random.default_rng(14)
+    dens = rng.normal(0, 5, 100000).astype(dtype)
+    dens[:2] = [1000.0, -1000.0]
+    itv = rng.uniform(0.1, 2.0, len(dens)).astype(dtype)
+    gb = rng.normal(size=len(dens)).astype(dtype)
+    rtol = 1e-5 if dtype == np.float32 else 1e-12
+    e, a = ru.raw2alpha(_t(dens, dev), -3.0, 0.5)
+    re, ra = oracle.raw2alpha(dens, -3.0, 0.5)
+    np.testing.assert_allclose(a.cpu().numpy(), ra, rtol=rtol, atol=1e-7)
+    np.testing.assert_allclose(e.cpu().numpy(), re, rtol=rtol)
+    e2, a2 = ru.raw2alpha_nonuni(_t(dens, dev), -3.0, _t(itv, dev))
+    re2, ra2 = oracle.raw2alpha_nonuni(dens, -3.0, itv)
+    np.testing.assert_allclose(a2.cpu().numpy(), ra2, rtol=rtol, atol=1e-7)
+    g = ru.raw2alpha_backward(e, _t(gb, dev), 0.5)
+    np.testing.assert_allclose(g.cpu().numpy(), oracle.raw2alpha_backward(re, gb, 0.5),
+                               rtol=10 * rtol, atol=1e-7)
+    g2 = ru.raw2alpha_nonuni_backward(e2, _t(gb, dev), _t(itv, dev))
+    np.testing.assert_allclose(g2.cpu().numpy(), oracle.raw2alpha_nonuni_backward(re2, gb, itv),
+                               rtol=10 * rtol, atol=1e-7)
+
+
+def _ray_segments(n_rays, seed, max_len=900):
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(1, max_len, n_rays)
+    lens[rng.random(n_rays) < 0.1] = 0
+    rid = np.repeat(np.arange(n_rays), lens).astype(np.int64)
+    return rid, rng
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_alpha2weight_bitexact(dev, dtype):
+    rid, rng = _ray_segments(4096, 15)
+    # mostly small alphas with a "surface" of large ones so many rays terminate
+    alpha = (rng.random(len(rid)) ** 8).astype(dtype)
+    alpha[rng.random(len(rid)) < 0.01] = 0.9
+    ref = oracle.alpha2weight(alpha, rid, 4100)
+    out = ru.alpha2weight(_t(alpha, dev), _t(rid, dev), 4100)
+    for name, a, b in zip(["weight", "T", "alphainv_last", "i_start", "i_end"], out, ref):
+        print(name)
+        _eq(a, b)
+    gw = rng.normal(size=len(rid)).astype(dtype)
+    gl = rng.normal(size=4100).astype(dtype)
+    g = ru.alpha2weight_backward(*out, 4100, _t(gw, dev), _t(gl, dev))
+    rg = oracle.alpha2weight_backward(alpha, *ref, 4100, gw, gl)
+    g = g.cpu().numpy()
+    scale = np.abs(rg).max()
+    np.testing.assert_allclose(g, rg, rtol=1e-4, atol=1e-6 * scale)

@@ -1,0 +1,283 @@
+"""Pins of the C oracle for the render_utils ops (CPU only).
+
+The reference ships no tests or golden vectors, so each oracle function is pinned here
+by what the mathematics fixes: closed forms recomputed independently in float64 numpy,
+brute force over hand-built cases with known answers, invariants of the flattened
+sample layout, and float64 autograd / finite differences for the adjoints.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from tests.raysets import edge_rays, mixed_rays
+
+BOX_MIN = np.array([-1.0, -1.0, -1.0])
+BOX_MAX = np.array([1.0, 1.0, 1.0])
+
+
+def slab_f64(o, d, near, far):
+    o = o.astype(np.float64)
+    d = d.astype(np.float64)
+    v = np.where(d == 0, 1e-6, d)
+    a = (BOX_MAX - o) / v
+    b = (BOX_MIN - o) / v
+    tmin = np.clip(np.minimum(a, b).max(-1), None, far)
+    tmin = np.maximum(tmin, near)
+    tmax = np.clip(np.maximum(a, b).min(-1), None, far)
+    tmax = np.maximum(tmax, near)
+    return tmin, tmax
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("near", [0.0, 0.2])
+def test_t_minmax_closed_form(dtype, near):
+    o, d = mixed_rays(400, seed=1, dtype=dtype)
+    tmin, tmax = oracle.infer_t_minmax(o, d, BOX_MIN.astype(dtype), BOX_MAX.astype(dtype), near, 1e9)
+    rmin, rmax = slab_f64(o, d, near, 1e9)
+    # float32 temporaries whatever the tensor dtype (render_utils_kernel.cu:23-33)
+    np.testing.assert_allclose(tmin, rmin, rtol=2e-6, atol=1e-6)
+    np.testing.assert_allclose(tmax, rmax, rtol=2e-6, atol=1e-6)
+
+
+def test_t_minmax_known_answers():
+    o, d = edge_rays(np.float32)
+    tmin, tmax = oracle.infer_t_minmax(o, d, BOX_MIN.astype(np.float32), BOX_MAX.astype(np.float32), 0.0, 1e9)
+    # axis-aligned ray from x=-3 along +x enters at x=-1 (t=2), leaves at x=1 (t=4)
+    assert (tmin[0], tmax[0]) == (2.0, 4.0)
+    # miss: both clamp to near
+    assert tmin[5] == 0.0 and tmax[5] == 0.0
+    # starts inside: t_min clamps to near
+    assert tmin[7] == 0.0 and tmax[7] > 0
+    # zero direction: 1e-6 substitution everywhere -> the slab is far away, clamps
+    assert np.isfinite(tmin[13]) and np.isfinite(tmax[13])
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_n_samples_brute_force(dtype):
+    o, d = mixed_rays(2000, seed=2, dtype=dtype)
+    step = np.float32(0.5 * 2 / 96)
+    tmin, tmax = oracle.infer_t_minmax(o, d, BOX_MIN.astype(dtype), BOX_MAX.astype(dtype), 0.0, 1e9)
+    ns = oracle.infer_n_samples(d, tmin, tmax, step)
+    q = (tmax.astype(np.float64) - tmin) * np.linalg.norm(d.astype(np.float64), axis=-1) / np.float64(step)
+    ref = np.maximum(np.ceil(q), 1).astype(np.int64)
+    tie = np.abs(q - np.round(q)) < 1e-4 * np.maximum(1, np.abs(q))
+    bad = ns != ref
+    assert not np.any(bad & ~tie), (ns[bad & ~tie], ref[bad & ~tie])
+    assert ns.min() >= 1
+    # the 1e-6 substitution is in t_minmax only; n_samples uses the raw |d| (quirk):
+    # the zero-direction ray therefore gets exactly one sample
+    assert ns[13] == 1
+
+
+def _float_pts(o, d, tmin, step, sid, dtype):
+    """pts in float64 from the closed form start + dir * step * k."""
+    o = o.astype(np.float64)
+    d = d.astype(np.float64)
+    nrm = np.linalg.norm(d, axis=-1, keepdims=True)
+    start = o + d * tmin[:, None]
+    return start, d / nrm
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_sample_pts_invariants(dtype):
+    o, d = mixed_rays(300, seed=3, dtype=dtype)
+    o, d = o[:-1], d[:-1]  # drop the zero-direction ray (NaN points) here
+    step = np.float32(0.5 * 2 / 96)
+    pts, mask, rid, sid, ns, tmin, tmax = oracle.sample_pts_on_rays(
+        o, d, BOX_MIN.astype(dtype), BOX_MAX.astype(dtype), 0.0, 1e9, step)
+    n = len(o)
+    assert ns.sum() == len(pts) == len(rid) == len(sid)
+    assert np.all(np.diff(rid) >= 0)
+    np.testing.assert_array_equal(np.bincount(rid, minlength=n), ns)
+    starts = np.concatenate([[0], np.cumsum(ns)[:-1]])
+    np.testing.assert_array_equal(sid, np.arange(len(sid)) - starts[rid])
+    start, dirn = _float_pts(o, d, tmin.astype(np.float64), step, sid, dtype)
+    ref = start[rid] + dirn[rid] * (np.float64(step) * sid)[:, None]
+    np.testing.assert_allclose(pts, ref, rtol=0, atol=2e-5 * 4)
+    # the mask is exactly the bbox test of the returned (float-rounded) points
+    out = np.any((pts < BOX_MIN.astype(dtype)) | (pts > BOX_MAX.astype(dtype)), axis=-1)
+    np.testing.assert_array_equal(mask, out)
+    # the segment of the axis-aligned ray: 2 / step samples, all inside but maybe the last
+    assert ns[0] == int(np.ceil(np.float32(2.0) / step))
+
+
+def test_sample_pts_miss_and_zero_dir():
+    o, d = edge_rays(np.float32)
+    step = np.float32(2 / 96 * 0.5)
+    pts, mask, rid, sid, ns, tmin, tmax = oracle.sample_pts_on_rays(
+        o, d, BOX_MIN.astype(np.float32), BOX_MAX.astype(np.float32), 0.0, 1e9, step)
+    miss = np.where(rid == 5)[0]
+    assert len(miss) == 1 and mask[miss[0]]  # one sample, outside the box
+    zero = np.where(rid == 13)[0]
+    assert len(zero) == 1 and np.all(np.isnan(pts[zero[0]])) and not mask[zero[0]]
+
+
+def test_sample_ndc_and_bg_closed_form():
+    o, d = mixed_rays(64, seed=4, dtype=np.float32)
+    o, d = o[:-1], d[:-1]
+    pts, mask = oracle.sample_ndc_pts_on_rays(o, d, BOX_MIN.astype(np.float32), BOX_MAX.astype(np.float32), 7)
+    k = np.arange(7) / 6.0
+    ref = o[:, None, :].astype(np.float64) + d[:, None, :].astype(np.float64) * k[None, :, None]
+    np.testing.assert_allclose(pts, ref, rtol=1e-6, atol=1e-6)
+    np.testing.assert_array_equal(mask, np.any((pts < -1) | (pts > 1), axis=-1))
+    # background points: inverted-sphere map of o + d t, t = t_max - 1 + 1/(1 - i/S)
+    tmax = np.abs(np.random.default_rng(0).normal(size=len(o))).astype(np.float32) + 1
+    bg = oracle.sample_bg_pts_on_rays(o, d, tmax, 0.3, 5)
+    i = np.arange(5)
+    t = tmax[:, None].astype(np.float64) - 1 + 1 / (1 - i / 5.0)
+    p = o[:, None, :].astype(np.float64) + d[:, None, :].astype(np.float64) * t[..., None]
+    tn = np.linalg.norm(p, axis=-1)
+    R = tn / np.abs(p).max(-1)
+    s = R ** 2 / tn ** 2 * 0.7 + R / tn * 0.3
+    np.testing.assert_allclose(bg, p * s[..., None], rtol=2e-5, atol=2e-5)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_maskcache_round_half_away(dtype):
+    rng = np.random.default_rng(5)
+    world = rng.random((4, 5, 6)) > 0.5
+    scale = np.array([1.0, 1.0, 1.0], dtype)
+    shift = np.array([0.0, 0.0, 0.0], dtype)
+    # exact .5 ties, negatives, out of range and NaN
+    pts = np.array([[0.5, 1.5, 2.5], [1.49, 2.5, 4.5], [-0.5, 0.0, 0.0], [-0.49, 0.0, 0.0],
+                    [3.5, 4.49, 5.49], [3.4, 4.5, 1.0], [np.nan, 1.0, 1.0], [1e10, 1, 1]], dtype)
+    out = oracle.maskcache_lookup(world, pts, scale, shift)
+    def rhaz(v):
+        return np.where(v >= 0, np.floor(v + 0.5), np.ceil(v - 0.5))
+    exp = []
+    for p in pts:
+        if np.any(np.isnan(p)):
+            ijk = [0 if np.isnan(c) else int(rhaz(c)) for c in p]  # NaN converts to index 0
+        else:
+            ijk = [int(rhaz(c)) for c in p]
+        ok = all(0 <= ijk[a] < world.shape[a] for a in range(3))
+        exp.append(bool(world[tuple(ijk)]) if ok else False)
+    np.testing.assert_array_equal(out, exp)
+    # 0.5 -> 1, 1.5 -> 2, 2.5 -> 3 (half away from zero, not to even)
+    assert out[0] == world[1, 2, 3]
+    # scaled / shifted lookup equals nearest-lattice lookup
+    q = rng.uniform(-1, 1, (500, 3)).astype(dtype)
+    sc = np.array([1.5, 2.0, 2.5], dtype)
+    sh = np.array([1.5, 2.0, 2.5], dtype)
+    out = oracle.maskcache_lookup(world, q, sc, sh)
+    v = (q * sc).astype(dtype) + sh
+    ijk = rhaz(v).astype(int)
+    ok = np.all((ijk >= 0) & (ijk < np.array(world.shape)), axis=-1)
+    exp = np.zeros(len(q), bool)
+    exp[ok] = world[ijk[ok, 0], ijk[ok, 1], ijk[ok, 2]]
+    np.testing.assert_array_equal(out, exp)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_raw2alpha_closed_form_and_fd(dtype):
+    rng = np.random.default_rng(6)
+    dens = rng.normal(0, 4, 1000).astype(dtype)
+    dens[:3] = [1000.0, -100.0, 0.0]
+    e, a = oracle.raw2alpha(dens, -2.0, 0.5)
+    ed = np.exp(dens.astype(np.float64) - 2.0)
+    ref = 1 - (1 + ed) ** (-0.5)
+    tol = 1e-6 if dtype == np.float32 else 1e-14
+    np.testing.assert_allclose(a, ref, rtol=tol * 4, atol=tol)
+    assert a[0] == 1.0 and np.isinf(e[0])  # e overflows to inf -> alpha 1
+    e2, a2 = oracle.raw2alpha_nonuni(dens, -2.0, np.full(len(dens), 0.5, dtype))
+    np.testing.assert_array_equal(a2, a)
+    # adjoint: d alpha / d density, checked by central differences in float64
+    if dtype == np.float64:
+        d = dens[3:200]
+        _, ap = oracle.raw2alpha(d + 1e-6, -2.0, 0.5)
+        _, am = oracle.raw2alpha(d - 1e-6, -2.0, 0.5)
+        fd = (ap - am) / 2e-6
+        ee, _ = oracle.raw2alpha(d, -2.0, 0.5)
+        g = oracle.raw2alpha_backward(ee, np.ones_like(d), 0.5)
+        np.testing.assert_allclose(g, fd, rtol=1e-6, atol=1e-9)
+    # e = inf: min(e, 1e10) * (1+e)^(-interval-1) = 0
+    g = oracle.raw2alpha_backward(e[:1], np.ones(1, dtype), 0.5)
+    assert g[0] == 0.0
+    # interval 0 -> alpha 0
+    _, a0 = oracle.raw2alpha(dens[3:10], 0.0, 0.0)
+    assert np.all(a0 == 0)
+
+
+def _segments(n_rays, rng, max_len=40, absent=0.2):
+    lens = rng.integers(1, max_len, n_rays)
+    lens[rng.random(n_rays) < absent] = 0
+    rid = np.repeat(np.arange(n_rays), lens)
+    return rid, lens
+
+
+def test_alpha2weight_brute_force_and_stop_rule():
+    rng = np.random.default_rng(7)
+    rid, lens = _segments(200, rng)
+    alpha = rng.random(len(rid)).astype(np.float32) ** 3
+    w, T, last, i_s, i_e = oracle.alpha2weight(alpha, rid, 200)
+    starts = np.concatenate([[0], np.cumsum(lens)[:-1]])
+    for r in range(200):
+        if lens[r] == 0:
+            assert last[r] == 1 and i_s[r] == 0 and i_e[r] == 0
+            continue
+        a = alpha[starts[r]:starts[r] + lens[r]]
+        # sequential recurrence, double product rounded to float at each step
+        t = np.float32(1)
+        stop = len(a)
+        Ts = []
+        for k, ak in enumerate(a):
+            Ts.append(t)
+            t = np.float32(np.float64(t) * (1.0 - np.float64(ak)))
+            if np.float64(t) < 1e-3:
+                stop = k + 1
+                break
+        assert i_s[r] == starts[r] and i_e[r] == starts[r] + stop
+        seg = slice(starts[r], starts[r] + stop)
+        np.testing.assert_array_equal(T[seg], np.array(Ts, np.float32))
+        np.testing.assert_array_equal(w[seg], np.array(Ts, np.float32) * a[:stop])
+        assert last[r] == t
+        # exact-product check: T_k = prod_{j<k} (1 - a_j) to float rounding
+        exact = np.concatenate([[1.0], np.cumprod(1 - a[:stop].astype(np.float64))[:-1]])
+        np.testing.assert_allclose(T[seg], exact, rtol=1e-5)
+        # entries past the stop keep weight 0, T 1
+        rest = slice(starts[r] + stop, starts[r] + lens[r])
+        assert np.all(w[rest] == 0) and np.all(T[rest] == 1)
+
+
+def test_alpha2weight_known_answers():
+    rid = np.array([0, 0, 0, 1, 1, 1, 3, 3], np.int64)
+    alpha = np.array([0, 0, 0, 1, 0.5, 0.5, 0.999, 0.5], np.float32)
+    w, T, last, i_s, i_e = oracle.alpha2weight(alpha, rid, 4)
+    np.testing.assert_array_equal(w[:3], 0)
+    np.testing.assert_array_equal(T[:3], 1)
+    assert last[0] == 1 and (i_s[0], i_e[0]) == (0, 3)
+    # alpha = 1 first: weight [1, 0, 0], stop right after it
+    np.testing.assert_array_equal(w[3:6], [1, 0, 0])
+    assert (i_s[1], i_e[1]) == (3, 4) and last[1] == 0
+    # absent ray 2
+    assert (i_s[2], i_e[2], last[2]) == (0, 0, 1)
+    # 1 - 0.999 in double, rounded to float, is ~1e-3: still >= 1e-3? compute
+    t1 = np.float32(np.float64(1.0) * (1 - np.float64(np.float32(0.999))))
+    assert (i_e[3] == 7) == (np.float64(t1) < 1e-3)
+    # empty input
+    w, T, last, i_s, i_e = oracle.alpha2weight(np.zeros(0, np.float32), np.zeros(0, np.int64), 3)
+    np.testing.assert_array_equal(last, 1)
+    np.testing.assert_array_equal(i_e, 0)
+
+
+def test_alpha2weight_backward_autograd():
+    rng = np.random.default_rng(8)
+    rid, lens = _segments(60, rng, max_len=30)
+    alpha = (rng.random(len(rid)) * 0.3).astype(np.float64)
+    w, T, last, i_s, i_e = oracle.alpha2weight(alpha, rid, 60)
+    gw = rng.normal(size=len(rid))
+    gl = rng.normal(size=60)
+    g = oracle.alpha2weight_backward(alpha, w, T, last, i_s, i_e, 60, gw, gl)
+    a_t = torch.tensor(alpha, requires_grad=True)
+    loss = 0
+    for r in range(60):
+        if i_e[r] <= i_s[r]:
+            continue
+        seg = a_t[i_s[r]:i_e[r]]
+        Tc = torch.cat([torch.ones(1, dtype=torch.float64), torch.cumprod(1 - seg, 0)])
+        loss = loss + (torch.tensor(gw[i_s[r]:i_e[r]]) * seg * Tc[:-1]).sum() + gl[r] * Tc[-1]
+    loss.backward()
+    # the reference's accumulator is float (render_utils_kernel.cu:671), so agreement
+    # is to float rounding even for double tensors
+    np.testing.assert_allclose(g, a_t.grad.numpy(), rtol=1e-5, atol=1e-6)

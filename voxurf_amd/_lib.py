@@ -1,0 +1,110 @@
+"""ctypes binding of libvoxhip.so.
+
+The argument types of every entry point are read from include/voxhip.h, the single
+declaration of the C ABI, so the binding and the header cannot drift apart.  torch is
+imported before the library is opened: torch's HIP runtime (SONAME libamdhip64.so.7)
+is then the one libvoxhip.so binds to, so device pointers and streams are shared.
+"""
+import ctypes
+import os
+import re
+import threading
+
+import torch  # noqa: F401  (must precede dlopen of libvoxhip.so; see module docstring)
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "_lib", "libvoxhip.so")
+HEADER_PATH = os.path.join(os.path.dirname(_PKG), "include", "voxhip.h")
+
+_CTYPE = {
+    "int": ctypes.c_int,
+    "int64_t": ctypes.c_int64,
+    "float": ctypes.c_float,
+    "double": ctypes.c_double,
+    "const char*": ctypes.c_char_p,
+    "void": None,
+}
+
+_DECL = re.compile(r"^\s*(const char\s*\*|int64_t|int|void)\s+(vx_\w+)\s*\(([^;]*?)\)\s*;",
+                   re.M | re.S)
+
+
+def parse_header(path=HEADER_PATH):
+    """Return {name: (restype_str, [argtype_str, ...])} for every vx_* declaration."""
+    text = open(path).read()
+    text = re.sub(r"/\*.*?\*/", " ", text, flags=re.S)
+    out = {}
+    for ret, name, args in _DECL.findall(text):
+        ret = re.sub(r"\s+", "", ret).replace("constchar*", "const char*")
+        types = []
+        args = args.strip()
+        if args and args != "void":
+            for a in args.split(","):
+                a = " ".join(a.split())
+                if "*" in a:
+                    types.append("ptr")
+                else:
+                    types.append(a.rsplit(" ", 1)[0])
+        out[name] = (ret, types)
+    return out
+
+
+def _argtype(t):
+    if t == "ptr":
+        return ctypes.c_void_p
+    return _CTYPE[t]
+
+
+_lock = threading.Lock()
+_lib = None
+
+
+class VoxhipError(RuntimeError):
+    pass
+
+
+def lib():
+    """Open libvoxhip.so once.  Raises if it has not been built: there is no fallback."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise VoxhipError(
+                    f"{LIB_PATH} is missing: build it with `python -m voxurf_amd.build` "
+                    "(the HIP library is required; there is no CPU fallback)")
+            h = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_GLOBAL)
+            for name, (ret, args) in parse_header().items():
+                fn = getattr(h, name)
+                fn.restype = _CTYPE[ret]
+                fn.argtypes = [_argtype(a) for a in args]
+            _lib = h
+    return _lib
+
+
+def call(name, *args):
+    """Call an int-returning entry point and raise RuntimeError on a non-zero status."""
+    rc = getattr(lib(), name)(*args)
+    if rc != 0:
+        msg = lib().vx_last_error().decode(errors="replace")
+        raise VoxhipError(f"{name} failed (status {rc}): {msg}")
+    return rc
+
+
+def ptr(t):
+    """Device address of a tensor (None -> NULL)."""
+    return None if t is None else t.data_ptr()
+
+
+def stream_of(t):
+    """hipStream_t of torch's current stream on the tensor's device."""
+    return torch.cuda.current_stream(t.device).cuda_stream
+
+
+def dtype_code(t):
+    if t.dtype == torch.float32:
+        return 0
+    if t.dtype == torch.float64:
+        return 1
+    raise VoxhipError(f"unsupported floating dtype {t.dtype} (float32 or float64 expected)")

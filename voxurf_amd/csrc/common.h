@@ -1,0 +1,81 @@
+// Shared device/host helpers for the voxurf_amd HIP library (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include "../../include/voxhip.h"
+
+namespace vx {
+
+// Thread-local text of the last failure; returned by vx_last_error().
+void set_error(const char* fmt, ...);
+
+constexpr int kWave = 64;
+
+// Launch-status helper: every entry point returns through this after its launches.
+inline int launch_status(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error("%s: kernel launch failed: %s", what, hipGetErrorString(e));
+    return VX_ERR_LAUNCH;
+  }
+  return VX_OK;
+}
+
+inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Grid size for a grid-stride loop over n items with `threads` per block; capped so
+// a launch never exceeds the hardware's grid limit while still filling 256 CUs.
+inline unsigned grid_for(int64_t n, int threads, int64_t cap = 1 << 20) {
+  int64_t b = cdiv(n > 0 ? n : 1, threads);
+  if (b > cap) b = cap;
+  return (unsigned)b;
+}
+
+#define VX_CHECK_ARG(cond, ...)          \
+  do {                                    \
+    if (!(cond)) {                        \
+      ::vx::set_error(__VA_ARGS__);       \
+      return VX_ERR_ARG;                  \
+    }                                     \
+  } while (0)
+
+// ---- device helpers -------------------------------------------------------------
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+// float -> int32 with the saturating semantics of a GPU cvt.rzi.s32.f32 instruction:
+// NaN -> 0, values beyond the int32 range clamp.  Used wherever an index is derived
+// from a float so that the mapping of out-of-range values is fixed.
+__device__ __forceinline__ int sat_f2i(float x) {
+  if (x != x) return 0;
+  if (x >= 2147483648.0f) return 2147483647;
+  if (x <= -2147483648.0f) return (-2147483647 - 1);
+  return (int)x;
+}
+__device__ __forceinline__ int sat_f2i(double x) {
+  if (x != x) return 0;
+  if (x >= 2147483648.0) return 2147483647;
+  if (x <= -2147483648.0) return (-2147483647 - 1);
+  return (int)x;
+}
+
+// Wave-wide inclusive prefix sum of a 64-bit integer (Hillis-Steele over 64 lanes).
+__device__ __forceinline__ int64_t wave_incl_scan_i64(int64_t v) {
+  const int l = lane_id();
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    int64_t u = __shfl_up(v, off, kWave);
+    if (l >= off) v += u;
+  }
+  return v;
+}
+
+__device__ __forceinline__ float wave_sum_f32(float v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+}  // namespace vx

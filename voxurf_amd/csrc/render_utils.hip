@@ -1,0 +1,696 @@
+// Ray sampling, mask lookup, raw->alpha and alpha->weight kernels (gfx950).
+//
+// Arithmetic follows the reference kernels in lib/cuda/render_utils_kernel.cu operation
+// by operation, including its mixed float/double temporaries, so that the integer
+// outputs (N_steps, ray_id, step_id, mask_outbbox, i_start/i_end, mask lookups) are
+// reproducible bit for bit.  Floating-point contraction is disabled in every kernel
+// whose results feed an integer output (no a*b+c fusion; IEEE division and sqrt),
+// which fixes the one rounding choice the reference source leaves to its compiler.
+#include <stdarg.h>
+#include <string.h>
+
+#include "common.h"
+
+namespace vx {
+
+static thread_local char g_err[512];
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+// ---------------------------------------------------------------------------------
+// Per-ray geometry (render_utils_kernel.cu:12-79).  T is the tensor scalar type; the
+// reference keeps several temporaries in `float` regardless of T, reproduced here.
+
+template <typename T>
+__device__ __forceinline__ void ray_t_minmax(const T* o, const T* d, const T* xyz_min,
+                                             const T* xyz_max, float near_, float far_,
+                                             float& tmin, float& tmax) {
+#pragma clang fp contract(off)
+  // `(d==0) ? 1e-6 : d` has type double in the reference; the result is stored in a float.
+  const float vx = (d[0] == 0) ? (float)1e-6 : (float)d[0];
+  const float vy = (d[1] == 0) ? (float)1e-6 : (float)d[1];
+  const float vz = (d[2] == 0) ? (float)1e-6 : (float)d[2];
+  const float ax = (float)((xyz_max[0] - o[0]) / (T)vx);
+  const float ay = (float)((xyz_max[1] - o[1]) / (T)vy);
+  const float az = (float)((xyz_max[2] - o[2]) / (T)vz);
+  const float bx = (float)((xyz_min[0] - o[0]) / (T)vx);
+  const float by = (float)((xyz_min[1] - o[1]) / (T)vy);
+  const float bz = (float)((xyz_min[2] - o[2]) / (T)vz);
+  tmin = fmaxf(fminf(fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fminf(az, bz)), far_), near_);
+  tmax = fmaxf(fminf(fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz)), far_), near_);
+}
+
+template <typename T>
+__device__ __forceinline__ float ray_norm(const T* d) {
+#pragma clang fp contract(off)
+  const T s = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  return (float)sqrt(s);
+}
+
+template <typename T>
+__device__ __forceinline__ int64_t ray_n_samples(const T* d, T tmin, T tmax, float stepdist) {
+#pragma clang fp contract(off)
+  const float rnorm = ray_norm(d);
+  const T q = (tmax - tmin) * (T)rnorm / (T)stepdist;
+  const double c = ceil((double)q);
+  return (int64_t)(c > 1.0 ? c : 1.0);
+}
+
+template <typename T>
+__global__ void k_infer_t_minmax(const T* __restrict__ rays_o, const T* __restrict__ rays_d,
+                                 const T* __restrict__ xyz_min, const T* __restrict__ xyz_max,
+                                 float near_, float far_, int64_t n, T* __restrict__ t_min,
+                                 T* __restrict__ t_max) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    float a, b;
+    ray_t_minmax<T>(rays_o + 3 * r, rays_d + 3 * r, xyz_min, xyz_max, near_, far_, a, b);
+    t_min[r] = (T)a;
+    t_max[r] = (T)b;
+  }
+}
+
+template <typename T>
+__global__ void k_infer_n_samples(const T* __restrict__ rays_d, const T* __restrict__ t_min,
+                                  const T* __restrict__ t_max, float stepdist, int64_t n,
+                                  int64_t* __restrict__ n_samples) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n;
+       r += (int64_t)gridDim.x * blockDim.x)
+    n_samples[r] = ray_n_samples<T>(rays_d + 3 * r, t_min[r], t_max[r], stepdist);
+}
+
+template <typename T>
+__device__ __forceinline__ void ray_start_dir(const T* o, const T* d, T tmin, T* start, T* dir) {
+#pragma clang fp contract(off)
+  const float rnorm = ray_norm(d);
+  start[0] = o[0] + d[0] * tmin;
+  start[1] = o[1] + d[1] * tmin;
+  start[2] = o[2] + d[2] * tmin;
+  dir[0] = d[0] / (T)rnorm;
+  dir[1] = d[1] / (T)rnorm;
+  dir[2] = d[2] / (T)rnorm;
+}
+
+template <typename T>
+__global__ void k_infer_ray_start_dir(const T* __restrict__ rays_o, const T* __restrict__ rays_d,
+                                      const T* __restrict__ t_min, int64_t n,
+                                      T* __restrict__ rays_start, T* __restrict__ rays_dir) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    T s[3], d[3];
+    ray_start_dir<T>(rays_o + 3 * r, rays_d + 3 * r, t_min[r], s, d);
+    for (int c = 0; c < 3; ++c) {
+      rays_start[3 * r + c] = s[c];
+      rays_dir[3 * r + c] = d[c];
+    }
+  }
+}
+
+// Phase 1 of sample_pts_on_rays: t_min/t_max and N_steps in one pass.
+template <typename T>
+__global__ void k_sample_count(const T* __restrict__ rays_o, const T* __restrict__ rays_d,
+                               const T* __restrict__ xyz_min, const T* __restrict__ xyz_max,
+                               float near_, float far_, float stepdist, int64_t n,
+                               T* __restrict__ t_min, T* __restrict__ t_max,
+                               int64_t* __restrict__ n_steps) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    float a, b;
+    ray_t_minmax<T>(rays_o + 3 * r, rays_d + 3 * r, xyz_min, xyz_max, near_, far_, a, b);
+    t_min[r] = (T)a;
+    t_max[r] = (T)b;
+    n_steps[r] = ray_n_samples<T>(rays_d + 3 * r, (T)a, (T)b, stepdist);
+  }
+}
+
+// Point of step `s` on a ray (render_utils_kernel.cu:184-192): the coordinates are
+// rounded to float whatever T is, and the bbox test compares in T.
+template <typename T>
+__device__ __forceinline__ bool sample_point(const T* start, const T* dir, const T* xyz_min,
+                                             const T* xyz_max, float stepdist, int64_t s,
+                                             float& px, float& py, float& pz) {
+#pragma clang fp contract(off)
+  const float dist = stepdist * (float)(int)s;
+  px = (float)(start[0] + dir[0] * (T)dist);
+  py = (float)(start[1] + dir[1] * (T)dist);
+  pz = (float)(start[2] + dir[2] * (T)dist);
+  return (xyz_min[0] > (T)px) | (xyz_min[1] > (T)py) | (xyz_min[2] > (T)pz) |
+         (xyz_max[0] < (T)px) | (xyz_max[1] < (T)py) | (xyz_max[2] < (T)pz);
+}
+
+// Phase 2: one wavefront per ray writes the ray's contiguous segment, so every store
+// of rays_pts / ray_id / step_id / mask is coalesced across the 64 lanes.
+template <typename T>
+__global__ void __launch_bounds__(256) k_sample_fill(
+    const T* __restrict__ rays_o, const T* __restrict__ rays_d, const T* __restrict__ xyz_min,
+    const T* __restrict__ xyz_max, const T* __restrict__ t_min, float stepdist, int64_t n,
+    const int64_t* __restrict__ csum, T* __restrict__ pts, uint8_t* __restrict__ mask,
+    int64_t* __restrict__ ray_id, int64_t* __restrict__ step_id) {
+  const int lane = threadIdx.x & 63;
+  const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+  for (int64_t r = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6); r < n;
+       r += waves) {
+    T start[3], dir[3];
+    ray_start_dir<T>(rays_o + 3 * r, rays_d + 3 * r, t_min[r], start, dir);
+    const int64_t beg = r ? csum[r - 1] : 0;
+    const int64_t cnt = csum[r] - beg;
+    for (int64_t s = lane; s < cnt; s += 64) {
+      float px, py, pz;
+      const bool out = sample_point<T>(start, dir, xyz_min, xyz_max, stepdist, s, px, py, pz);
+      const int64_t m = beg + s;
+      pts[3 * m + 0] = (T)px;
+      pts[3 * m + 1] = (T)py;
+      pts[3 * m + 2] = (T)pz;
+      mask[m] = out;
+      ray_id[m] = r;
+      step_id[m] = s;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// Inclusive int64 scan: tiles of 1024 scanned per block, tile totals scanned by one
+// block, then offsets added.  Workspace: one int64 per tile.
+
+constexpr int kScanThreads = 256;
+constexpr int kScanTile = 1024;
+
+__device__ int64_t block_excl_scan(int64_t v, int64_t* lds, int64_t& total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int64_t inc = wave_incl_scan_i64(v);
+  if (lane == 63) lds[w] = inc;
+  __syncthreads();
+  int64_t off = 0;
+  total = 0;
+  for (int i = 0; i < kScanThreads / 64; ++i) {
+    if (i < w) off += lds[i];
+    total += lds[i];
+  }
+  __syncthreads();
+  return off + inc - v;
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_tiles(const int64_t* __restrict__ in,
+                                                             int64_t* __restrict__ out,
+                                                             int64_t n, int64_t* __restrict__ tile_sum) {
+  __shared__ int64_t lds[kScanThreads / 64];
+  const int64_t base = (int64_t)blockIdx.x * kScanTile + threadIdx.x * 4;
+  int64_t v[4];
+  int64_t s = 0;
+  for (int i = 0; i < 4; ++i) {
+    v[i] = (base + i < n) ? in[base + i] : 0;
+    s += v[i];
+  }
+  int64_t total;
+  int64_t off = block_excl_scan(s, lds, total);
+  for (int i = 0; i < 4; ++i) {
+    off += v[i];
+    if (base + i < n) out[base + i] = off;
+  }
+  if (threadIdx.x == 0) tile_sum[blockIdx.x] = total;
+}
+
+// Single block: exclusive scan of the tile totals in place; optionally writes the grand
+// total to *total_out.
+__global__ void __launch_bounds__(kScanThreads) k_scan_sums(int64_t* __restrict__ tile_sum,
+                                                            int64_t ntiles,
+                                                            int64_t* __restrict__ total_out) {
+  __shared__ int64_t lds[kScanThreads / 64];
+  int64_t carry = 0;
+  for (int64_t b = 0; b < ntiles; b += kScanThreads) {
+    const int64_t i = b + threadIdx.x;
+    const int64_t v = (i < ntiles) ? tile_sum[i] : 0;
+    int64_t total;
+    const int64_t ex = block_excl_scan(v, lds, total);
+    if (i < ntiles) tile_sum[i] = carry + ex;
+    carry += total;
+  }
+  if (threadIdx.x == 0 && total_out) *total_out = carry;
+}
+
+__global__ void k_scan_add(int64_t* __restrict__ out, int64_t n,
+                           const int64_t* __restrict__ tile_off) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] += tile_off[i / kScanTile];
+}
+
+template <typename T>
+__global__ void k_fill(T* p, int64_t n, T v);
+
+static int scan_i64(const int64_t* in, int64_t* out, int64_t n, int64_t* ws, int64_t* total,
+                    hipStream_t st) {
+  if (n <= 0) {
+    if (total) k_fill<int64_t><<<1, 64, 0, st>>>(total, 1, (int64_t)0);
+    return launch_status("scan");
+  }
+  const int64_t ntiles = cdiv(n, kScanTile);
+  k_scan_tiles<<<(unsigned)ntiles, kScanThreads, 0, st>>>(in, out, n, ws);
+  k_scan_sums<<<1, kScanThreads, 0, st>>>(ws, ntiles, total);
+  if (ntiles > 1) k_scan_add<<<(unsigned)cdiv(n, 256), 256, 0, st>>>(out, n, ws);
+  return launch_status("scan");
+}
+
+// ---------------------------------------------------------------------------------
+// Fixed-count NDC and background sampling (render_utils_kernel.cu:244-360).
+
+template <typename T>
+__global__ void k_sample_ndc(const T* __restrict__ rays_o, const T* __restrict__ rays_d,
+                             const T* __restrict__ xyz_min, const T* __restrict__ xyz_max,
+                             int ns, int64_t n, T* __restrict__ pts, uint8_t* __restrict__ mask) {
+#pragma clang fp contract(off)
+  const int64_t total = n * ns;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = idx / ns;
+    const int s = (int)(idx % ns);
+    const float dist = ((float)s) / (float)(ns - 1);
+    const T* o = rays_o + 3 * r;
+    const T* d = rays_d + 3 * r;
+    const float px = (float)(o[0] + d[0] * (T)dist);
+    const float py = (float)(o[1] + d[1] * (T)dist);
+    const float pz = (float)(o[2] + d[2] * (T)dist);
+    pts[3 * idx + 0] = (T)px;
+    pts[3 * idx + 1] = (T)py;
+    pts[3 * idx + 2] = (T)pz;
+    mask[idx] = (xyz_min[0] > (T)px) | (xyz_min[1] > (T)py) | (xyz_min[2] > (T)pz) |
+                (xyz_max[0] < (T)px) | (xyz_max[1] < (T)py) | (xyz_max[2] < (T)pz);
+  }
+}
+
+template <typename T>
+__global__ void k_sample_bg(const T* __restrict__ rays_o, const T* __restrict__ rays_d,
+                            const T* __restrict__ t_max, float bg_preserve, int ns, int64_t n,
+                            T* __restrict__ pts) {
+#pragma clang fp contract(off)
+  const int64_t total = n * ns;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = idx / ns;
+    const int s = (int)(idx % ns);
+    const T* o = rays_o + 3 * r;
+    const T* d = rays_d + 3 * r;
+    const float t_inner = (float)t_max[r];
+    const float frac = ((float)s) / (float)ns;
+    const float t_outer0 = (float)((double)t_inner - 1. + 1. / (1. - (double)frac));
+    const float x = (float)(o[0] + d[0] * (T)t_outer0);
+    const float y = (float)(o[1] + d[1] * (T)t_outer0);
+    const float z = (float)(o[2] + d[2] * (T)t_outer0);
+    const float t_outer = sqrtf(x * x + y * y + z * z);
+    const float m = fmaxf(fabsf(x), fmaxf(fabsf(y), fabsf(z)));
+    const float R = t_outer / m;
+    const float o2i = (float)((double)(R * R / (t_outer * t_outer)) * (1. - (double)bg_preserve) +
+                              (double)(R / t_outer * bg_preserve));
+    pts[3 * idx + 0] = (T)(x * o2i);
+    pts[3 * idx + 1] = (T)(y * o2i);
+    pts[3 * idx + 2] = (T)(z * o2i);
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// maskcache_lookup (render_utils_kernel.cu:367-392): nearest voxel by round-half-away.
+
+template <typename T>
+__global__ void k_maskcache(const uint8_t* __restrict__ world, const T* __restrict__ xyz,
+                            const T* __restrict__ scale, const T* __restrict__ shift, int si,
+                            int sj, int sk, int64_t n, uint8_t* __restrict__ out) {
+#pragma clang fp contract(off)
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    const T* x = xyz + 3 * p;
+    const int i = sat_f2i(round(x[0] * scale[0] + shift[0]));
+    const int j = sat_f2i(round(x[1] * scale[1] + shift[1]));
+    const int k = sat_f2i(round(x[2] * scale[2] + shift[2]));
+    uint8_t v = 0;
+    if (0 <= i && i < si && 0 <= j && j < sj && 0 <= k && k < sk)
+      v = world[((int64_t)i * sj + j) * sk + k] != 0;
+    out[p] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// raw2alpha family (render_utils_kernel.cu:431-530).
+
+template <typename T>
+__global__ void k_raw2alpha(const T* __restrict__ density, float shift, float interval,
+                            const T* __restrict__ interval_arr, int64_t n, T* __restrict__ exp_d,
+                            T* __restrict__ alpha) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const T itv = interval_arr ? interval_arr[i] : (T)interval;
+    const T e = exp(density[i] + (T)shift);
+    exp_d[i] = e;
+    alpha[i] = (T)1 - pow((T)1 + e, -itv);
+  }
+}
+
+template <typename T>
+__global__ void k_raw2alpha_bwd(const T* __restrict__ exp_d, const T* __restrict__ grad_back,
+                                float interval, const T* __restrict__ interval_arr, int64_t n,
+                                T* __restrict__ grad) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const T itv = interval_arr ? interval_arr[i] : (T)interval;
+    const T e = exp_d[i];
+    // min(e, 1e10) has type double in the reference; the product is formed left to right.
+    const double ec = fmin((double)e, 1e10);
+    const T pw = pow((T)1 + e, -itv - (T)1);
+    grad[i] = (T)(((ec * (double)pw) * (double)itv) * (double)grad_back[i]);
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// alpha2weight (render_utils_kernel.cu:577-651).  The transmittance recurrence
+//   T <- float( double(T) * (1.0 - double(alpha)) ),  stop after T < 1e-3
+// is inherently sequential at float rounding, so each ray's segment is walked by one
+// wavefront: 64 alphas are loaded coalesced, the recurrence is stepped on uniform
+// values, and each lane picks up the T of its own sample.
+
+template <typename T>
+__global__ void k_a2w_init(int64_t n_pts, int64_t n_rays, T* __restrict__ weight,
+                           T* __restrict__ Tt, T* __restrict__ last, int64_t* __restrict__ i_start,
+                           int64_t* __restrict__ i_end) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n_pts; i += stride) {
+    weight[i] = 0;
+    Tt[i] = 1;
+  }
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n_rays; r += stride) {
+    last[r] = 1;
+    i_start[r] = 0;
+    i_end[r] = 0;
+  }
+}
+
+// render_utils_kernel.cu:607-617 plus the host-side `i_end[ray_id[n_pts-1]] = n_pts`
+// (:635), done on the device.  Ray ids outside [0, n_rays) are ignored.
+__global__ void k_a2w_bounds(const int64_t* __restrict__ ray_id, int64_t n_pts, int64_t n_rays,
+                             int64_t* __restrict__ i_start, int64_t* __restrict__ i_end) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x + 1; i < n_pts;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t a = ray_id[i - 1], b = ray_id[i];
+    if (a != b) {
+      if (0 <= b && b < n_rays) i_start[b] = i;
+      if (0 <= a && a < n_rays) i_end[a] = i;
+    }
+  }
+}
+
+__global__ void k_a2w_last(const int64_t* __restrict__ ray_id, int64_t n_pts, int64_t n_rays,
+                           int64_t* __restrict__ i_end) {
+  const int64_t r = ray_id[n_pts - 1];
+  if (0 <= r && r < n_rays) i_end[r] = n_pts;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_a2w_scan(const T* __restrict__ alpha, int64_t n_rays,
+                                                  int64_t n_pts, T* __restrict__ weight,
+                                                  T* __restrict__ Tt, T* __restrict__ last,
+                                                  const int64_t* __restrict__ i_start,
+                                                  int64_t* __restrict__ i_end) {
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x & 63;
+  const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+  for (int64_t r = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6); r < n_rays;
+       r += waves) {
+    const int64_t is = i_start[r];
+    int64_t ie = i_end[r];
+    if (ie > n_pts) ie = n_pts;
+    float Tc = 1.f;
+    int64_t stop = ie;
+    for (int64_t base = is; base < ie; base += 64) {
+      const int64_t i = base + lane;
+      const T a = (i < ie) ? alpha[i] : (T)0;
+      const int cnt = (int)((ie - base) < 64 ? (ie - base) : 64);
+      float myT = 1.f;
+      bool done = false;
+      for (int k = 0; k < cnt; ++k) {
+        const T ak = __shfl(a, k, 64);
+        if (lane == k) myT = Tc;
+        Tc = (float)((double)Tc * (1.0 - (double)ak));
+        if ((double)Tc < 1e-3) {
+          stop = base + k + 1;
+          done = true;
+          break;
+        }
+      }
+      if (i < stop && i < ie) {
+        Tt[i] = (T)myT;
+        weight[i] = (T)myT * a;
+      }
+      if (done) break;
+    }
+    if (lane == 0) {
+      i_end[r] = stop < is ? is : stop;
+      last[r] = (T)Tc;
+    }
+  }
+}
+
+// alpha2weight_backward (render_utils_kernel.cu:654-677):
+//   grad_i = gw_i*T_i - back_i / (1 - alpha_i + 1e-10),
+//   back_i = grad_last*T_last + sum_{j>i, j<i_end} gw_j*w_j
+// The suffix sums are formed per 64-sample chunk with a wave scan, walking the ray's
+// segment from its end.
+template <typename T>
+__global__ void __launch_bounds__(256) k_a2w_bwd(
+    const T* __restrict__ alpha, const T* __restrict__ weight, const T* __restrict__ Tt,
+    const T* __restrict__ last, const int64_t* __restrict__ i_start,
+    const int64_t* __restrict__ i_end, int64_t n_rays, int64_t n_pts,
+    const T* __restrict__ gw, const T* __restrict__ glast, T* __restrict__ grad) {
+  const int lane = threadIdx.x & 63;
+  const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+  for (int64_t r = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6); r < n_rays;
+       r += waves) {
+    const int64_t is = i_start[r];
+    int64_t ie = i_end[r];
+    if (ie > n_pts) ie = n_pts;
+    float back = (float)(glast[r] * last[r]);
+    for (int64_t top = ie; top > is; top -= 64) {
+      const int64_t lo = (top - 64 > is) ? top - 64 : is;
+      const int64_t i = lo + lane;
+      const bool valid = i < top;
+      const T c = valid ? gw[i] * weight[i] : (T)0;
+      // suffix sum over lanes > this lane within the chunk
+      T incl = c;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const T u = __shfl_down(incl, off, 64);
+        if (lane + off < 64) incl += u;
+      }
+      const T excl = incl - c;
+      if (valid) {
+        const T b = (T)back + excl;
+        const double den = (double)((T)1 - alpha[i]) + 1e-10;
+        grad[i] = (T)((double)(gw[i] * Tt[i]) - (double)(float)b / den);
+      }
+      back = (float)((T)back + __shfl(incl, 0, 64));
+    }
+  }
+}
+
+__global__ void k_fill_zero_bytes(uint8_t* p, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
+template <typename T>
+__global__ void k_fill(T* p, int64_t n, T v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+}  // namespace vx
+
+using namespace vx;
+
+#define VX_DISPATCH(dtype, ...)                                          \
+  do {                                                                   \
+    if ((dtype) == VX_F32) {                                             \
+      using scalar_t = float;                                            \
+      __VA_ARGS__;                                                       \
+    } else if ((dtype) == VX_F64) {                                      \
+      using scalar_t = double;                                           \
+      __VA_ARGS__;                                                       \
+    } else {                                                             \
+      vx::set_error("unsupported dtype code %d", (int)(dtype));          \
+      return VX_ERR_DTYPE;                                               \
+    }                                                                    \
+  } while (0)
+
+#define ST(p) ((hipStream_t)(p))
+#define CP(p) ((const scalar_t*)(p))
+#define MP(p) ((scalar_t*)(p))
+
+extern "C" {
+
+const char* vx_last_error(void) { return vx::g_err; }
+int vx_version(void) { return 1; }
+
+int vx_infer_t_minmax(int dtype, const void* rays_o, const void* rays_d, const void* xyz_min,
+                      const void* xyz_max, float near_, float far_, int64_t n, void* t_min,
+                      void* t_max, void* stream) {
+  VX_CHECK_ARG(n >= 0, "infer_t_minmax: n_rays < 0");
+  if (n == 0) return VX_OK;
+  VX_DISPATCH(dtype, k_infer_t_minmax<scalar_t><<<grid_for(n, 256), 256, 0, ST(stream)>>>(
+                         CP(rays_o), CP(rays_d), CP(xyz_min), CP(xyz_max), near_, far_, n,
+                         MP(t_min), MP(t_max)));
+  return launch_status("infer_t_minmax");
+}
+
+int vx_infer_n_samples(int dtype, const void* rays_d, const void* t_min, const void* t_max,
+                       float stepdist, int64_t n, int64_t* n_samples, void* stream) {
+  VX_CHECK_ARG(n >= 0, "infer_n_samples: n_rays < 0");
+  if (n == 0) return VX_OK;
+  VX_DISPATCH(dtype, k_infer_n_samples<scalar_t><<<grid_for(n, 256), 256, 0, ST(stream)>>>(
+                         CP(rays_d), CP(t_min), CP(t_max), stepdist, n, n_samples));
+  return launch_status("infer_n_samples");
+}
+
+int vx_infer_ray_start_dir(int dtype, const void* rays_o, const void* rays_d, const void* t_min,
+                           int64_t n, void* rays_start, void* rays_dir, void* stream) {
+  VX_CHECK_ARG(n >= 0, "infer_ray_start_dir: n_rays < 0");
+  if (n == 0) return VX_OK;
+  VX_DISPATCH(dtype, k_infer_ray_start_dir<scalar_t><<<grid_for(n, 256), 256, 0, ST(stream)>>>(
+                         CP(rays_o), CP(rays_d), CP(t_min), n, MP(rays_start), MP(rays_dir)));
+  return launch_status("infer_ray_start_dir");
+}
+
+int64_t vx_scan_workspace_elems(int64_t n) { return n > 0 ? cdiv(n, kScanTile) : 1; }
+
+int vx_inclusive_scan_i64(const int64_t* in, int64_t* out, int64_t n, int64_t* ws, void* stream) {
+  VX_CHECK_ARG(n >= 0, "inclusive_scan: n < 0");
+  return scan_i64(in, out, n, ws, nullptr, ST(stream));
+}
+
+int vx_sample_pts_count(int dtype, const void* rays_o, const void* rays_d, const void* xyz_min,
+                        const void* xyz_max, float near_, float far_, float stepdist, int64_t n,
+                        void* t_min, void* t_max, int64_t* n_steps, int64_t* csum,
+                        int64_t* total, int64_t* ws, void* stream) {
+  VX_CHECK_ARG(n >= 0, "sample_pts_on_rays: n_rays < 0");
+  if (n > 0)
+    VX_DISPATCH(dtype, k_sample_count<scalar_t><<<grid_for(n, 256), 256, 0, ST(stream)>>>(
+                           CP(rays_o), CP(rays_d), CP(xyz_min), CP(xyz_max), near_, far_,
+                           stepdist, n, MP(t_min), MP(t_max), n_steps));
+  int st = launch_status("sample_pts_on_rays(count)");
+  if (st) return st;
+  return scan_i64(n_steps, csum, n, ws, total, ST(stream));
+}
+
+int vx_sample_pts_fill(int dtype, const void* rays_o, const void* rays_d, const void* xyz_min,
+                       const void* xyz_max, const void* t_min, float stepdist, int64_t n,
+                       const int64_t* csum, int64_t total, void* pts, uint8_t* mask,
+                       int64_t* ray_id, int64_t* step_id, void* stream) {
+  VX_CHECK_ARG(n >= 0 && total >= 0, "sample_pts_on_rays: negative size");
+  if (n == 0 || total == 0) return VX_OK;
+  VX_DISPATCH(dtype, k_sample_fill<scalar_t><<<grid_for(n, 4, 1 << 16), 256, 0, ST(stream)>>>(
+                         CP(rays_o), CP(rays_d), CP(xyz_min), CP(xyz_max), CP(t_min), stepdist, n,
+                         csum, MP(pts), mask, ray_id, step_id));
+  return launch_status("sample_pts_on_rays(fill)");
+}
+
+int vx_sample_ndc_pts_on_rays(int dtype, const void* rays_o, const void* rays_d,
+                              const void* xyz_min, const void* xyz_max, int ns, int64_t n,
+                              void* pts, uint8_t* mask, void* stream) {
+  VX_CHECK_ARG(n >= 0 && ns >= 0, "sample_ndc_pts_on_rays: negative size");
+  if (n == 0 || ns == 0) return VX_OK;
+  VX_DISPATCH(dtype, k_sample_ndc<scalar_t><<<grid_for(n * ns, 256), 256, 0, ST(stream)>>>(
+                         CP(rays_o), CP(rays_d), CP(xyz_min), CP(xyz_max), ns, n, MP(pts), mask));
+  return launch_status("sample_ndc_pts_on_rays");
+}
+
+int vx_sample_bg_pts_on_rays(int dtype, const void* rays_o, const void* rays_d, const void* t_max,
+                             float bg_preserve, int ns, int64_t n, void* pts, void* stream) {
+  VX_CHECK_ARG(n >= 0 && ns >= 0, "sample_bg_pts_on_rays: negative size");
+  if (n == 0 || ns == 0) return VX_OK;
+  VX_DISPATCH(dtype, k_sample_bg<scalar_t><<<grid_for(n * ns, 256), 256, 0, ST(stream)>>>(
+                         CP(rays_o), CP(rays_d), CP(t_max), bg_preserve, ns, n, MP(pts)));
+  return launch_status("sample_bg_pts_on_rays");
+}
+
+int vx_maskcache_lookup(int dtype, const uint8_t* world, const void* xyz, const void* scale,
+                        const void* shift, int si, int sj, int sk, int64_t n, uint8_t* out,
+                        void* stream) {
+  VX_CHECK_ARG(n >= 0 && si >= 0 && sj >= 0 && sk >= 0, "maskcache_lookup: negative size");
+  if (n == 0) return VX_OK;
+  VX_DISPATCH(dtype, k_maskcache<scalar_t><<<grid_for(n, 256), 256, 0, ST(stream)>>>(
+                         world, CP(xyz), CP(scale), CP(shift), si, sj, sk, n, out));
+  return launch_status("maskcache_lookup");
+}
+
+int vx_raw2alpha(int dtype, const void* density, float shift, float interval, int64_t n,
+                 void* exp_d, void* alpha, void* stream) {
+  VX_CHECK_ARG(n >= 0, "raw2alpha: n < 0");
+  if (n == 0) return VX_OK;
+  VX_DISPATCH(dtype, k_raw2alpha<scalar_t><<<grid_for(n, 256), 256, 0, ST(stream)>>>(
+                         CP(density), shift, interval, nullptr, n, MP(exp_d), MP(alpha)));
+  return launch_status("raw2alpha");
+}
+
+int vx_raw2alpha_nonuni(int dtype, const void* density, float shift, const void* interval,
+                        int64_t n, void* exp_d, void* alpha, void* stream) {
+  VX_CHECK_ARG(n >= 0, "raw2alpha_nonuni: n < 0");
+  if (n == 0) return VX_OK;
+  VX_DISPATCH(dtype, k_raw2alpha<scalar_t><<<grid_for(n, 256), 256, 0, ST(stream)>>>(
+                         CP(density), shift, 0.f, CP(interval), n, MP(exp_d), MP(alpha)));
+  return launch_status("raw2alpha_nonuni");
+}
+
+int vx_raw2alpha_backward(int dtype, const void* exp_d, const void* grad_back, float interval,
+                          int64_t n, void* grad, void* stream) {
+  VX_CHECK_ARG(n >= 0, "raw2alpha_backward: n < 0");
+  if (n == 0) return VX_OK;
+  VX_DISPATCH(dtype, k_raw2alpha_bwd<scalar_t><<<grid_for(n, 256), 256, 0, ST(stream)>>>(
+                         CP(exp_d), CP(grad_back), interval, nullptr, n, MP(grad)));
+  return launch_status("raw2alpha_backward");
+}
+
+int vx_raw2alpha_nonuni_backward(int dtype, const void* exp_d, const void* grad_back,
+                                 const void* interval, int64_t n, void* grad, void* stream) {
+  VX_CHECK_ARG(n >= 0, "raw2alpha_nonuni_backward: n < 0");
+  if (n == 0) return VX_OK;
+  VX_DISPATCH(dtype, k_raw2alpha_bwd<scalar_t><<<grid_for(n, 256), 256, 0, ST(stream)>>>(
+                         CP(exp_d), CP(grad_back), 0.f, CP(interval), n, MP(grad)));
+  return launch_status("raw2alpha_nonuni_backward");
+}
+
+int vx_alpha2weight(int dtype, const void* alpha, const int64_t* ray_id, int64_t n_pts,
+                    int64_t n_rays, void* weight, void* T, void* last, int64_t* i_start,
+                    int64_t* i_end, void* stream) {
+  VX_CHECK_ARG(n_pts >= 0 && n_rays >= 0, "alpha2weight: negative size");
+  hipStream_t st = ST(stream);
+  if (n_pts + n_rays == 0) return VX_OK;
+  VX_DISPATCH(dtype, k_a2w_init<scalar_t><<<grid_for(n_pts > n_rays ? n_pts : n_rays, 256), 256,
+                                            0, st>>>(n_pts, n_rays, MP(weight), MP(T), MP(last),
+                                                     i_start, i_end));
+  if (n_pts == 0 || n_rays == 0) return launch_status("alpha2weight");
+  k_a2w_bounds<<<grid_for(n_pts, 256), 256, 0, st>>>(ray_id, n_pts, n_rays, i_start, i_end);
+  k_a2w_last<<<1, 1, 0, st>>>(ray_id, n_pts, n_rays, i_end);
+  VX_DISPATCH(dtype, k_a2w_scan<scalar_t><<<grid_for(n_rays, 4, 1 << 16), 256, 0, st>>>(
+                         CP(alpha), n_rays, n_pts, MP(weight), MP(T), MP(last), i_start, i_end));
+  return launch_status("alpha2weight");
+}
+
+int vx_alpha2weight_backward(int dtype, const void* alpha, const void* weight, const void* T,
+                             const void* last, const int64_t* i_start, const int64_t* i_end,
+                             int64_t n_pts, int64_t n_rays, const void* gw, const void* glast,
+                             void* grad, void* stream) {
+  VX_CHECK_ARG(n_pts >= 0 && n_rays >= 0, "alpha2weight_backward: negative size");
+  hipStream_t st = ST(stream);
+  if (n_pts == 0) return VX_OK;
+  VX_DISPATCH(dtype, k_fill<scalar_t><<<grid_for(n_pts, 256), 256, 0, st>>>(MP(grad), n_pts,
+                                                                            (scalar_t)0));
+  if (n_rays == 0) return launch_status("alpha2weight_backward");
+  VX_DISPATCH(dtype, k_a2w_bwd<scalar_t><<<grid_for(n_rays, 4, 1 << 16), 256, 0, st>>>(
+                         CP(alpha), CP(weight), CP(T), CP(last), i_start, i_end, n_rays, n_pts,
+                         CP(gw), CP(glast), MP(grad)));
+  return launch_status("alpha2weight_backward");
+}
+
+}  // extern "C"
