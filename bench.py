@@ -1,0 +1,167 @@
+"""Fine training-step throughput on one MI355X (rays/s), with HBM roofline and the
+CPU restatement timed beside it.
+
+Workload: the plain fine step -- 256^3 float32 SDF grid (analytic perturbed sphere) +
+256^3 x 12 float32 feature grid ~N(0, 0.1^2), rgbnet 42->192->192->192->3, 8192 rays per
+step from the radius-3 sphere, stepsize 0.5 voxel, s = 256, forward + backward + Adam
+(lr sdf 5e-3, k0 0.1, rgbnet 3e-3).  A pool of ray batches is generated once and kept
+in HBM; each step takes the next batch.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--path fused|ops] [--no-cpu]
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# Peak figures of an MI355X used for roofline fractions.
+PEAK_HBM_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--path", choices=["fused", "ops"], default="fused")
+    ap.add_argument("--mlp", choices=["fp32", "bf16"], default="fp32")
+    ap.add_argument("--rays", type=int, default=8192)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    return ap.parse_args()
+
+
+def step_bytes(cfg, counts):
+    """Algorithmic HBM bytes of one fine step (read/write of every grid element by Adam
+    and the gradient clear, ray I/O, and the trilinear corner traffic of the gathers and
+    scatters), from the per-step sample counts."""
+    R, C = cfg.R, cfg.feat_dim
+    n_e = R ** 3 * (1 + C)
+    m0, m1b, m2 = counts["m0"], counts["m1b"], counts["m2"]
+    return (32 * n_e + 60 * cfg.n_rays + m0 * 7 * 8 * 4 + m2 * 8 * C * 4 + m1b * 7 * 8 * 8
+            + m2 * 8 * C * 8)
+
+
+def cpu_baseline(cfg, threads):
+    import torch
+    from oracle import torch_ref as tr
+    from voxurf_amd import scene
+    torch.set_num_threads(threads)
+    st = tr.make_state(cfg)
+    o, d, v = scene.rays(cfg.n_rays, seed=cfg.seed)
+    tg = scene.targets(cfg.n_rays, seed=cfg.seed)
+    opt = {}
+    tr.train_step(st, opt, o, d, v, tg, cfg, 1)  # warm-up (allocates optimiser state)
+    t0 = time.perf_counter()
+    tr.train_step(st, opt, o, d, v, tg, cfg, 2)
+    dt = time.perf_counter() - t0
+    return dict(value=cfg.n_rays / dt, unit="rays/s", cores=threads, kind="port",
+                sample=f"1 full fine step (8192 rays, 256^3 grids) of the PyTorch-CPU "
+                       f"restatement after 1 warm-up step, {dt:.2f} s")
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 "
+                                   "(multi-GPU sharding is a design note only)"}))
+        sys.exit(2)
+    import torch
+    from voxurf_amd import scene
+    from voxurf_amd.fused import FusedFineTrainer
+    from voxurf_amd.model import Trainer, VoxurfFine
+
+    cfg = scene.FINE
+    if args.rays != cfg.n_rays:
+        import dataclasses
+        cfg = dataclasses.replace(cfg, n_rays=args.rays)
+    dev = torch.device("cuda:0")
+    mlp_dtype = torch.bfloat16 if args.mlp == "bf16" else torch.float32
+
+    n_pool = 4
+    pool = []
+    for i in range(n_pool):
+        o, d, v = scene.rays(cfg.n_rays, seed=100 + i)
+        pool.append((o.to(dev), d.to(dev), v.to(dev), scene.targets(cfg.n_rays, seed=100 + i).to(dev)))
+
+    sdf = scene.sdf_grid(cfg.R).to(dev)
+    k0 = scene.feature_grid(cfg.R, cfg.feat_dim).to(dev)
+    mlp = scene.mlp_params(cfg.mlp_in, cfg.mlp_width, cfg.mlp_depth)
+    if args.path == "fused":
+        trainer = FusedFineTrainer(cfg, sdf, k0, mlp, device=dev, mlp_dtype=mlp_dtype)
+    else:
+        model = VoxurfFine(cfg, sdf, k0, mlp, mlp_dtype=mlp_dtype).to(dev)
+        trainer = Trainer(model, cfg)
+    del sdf, k0
+
+    def run(i, **kw):
+        o, d, v, t = pool[i % n_pool]
+        return trainer.step(o, d, v, t, **kw)
+
+    for i in range(args.warmup):
+        run(i)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    t0 = time.perf_counter()
+    ev[0].record(stream)
+    for i in range(args.steps):
+        run(args.warmup + i)
+        ev[i + 1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    per = [ev[i].elapsed_time(ev[i + 1]) for i in range(args.steps)]
+    per.sort()
+    ms = sum(per) / len(per)
+
+    # Per-kernel timing of the dominant kernel (the Adam sweep over the 256^3 x 12
+    # feature grid) with HIP events on the stream it is launched on.
+    out = run(args.warmup + args.steps, profile=True)
+    torch.cuda.synchronize()
+    adam_ms = trainer.last_profile["adam_k0_ms"]
+    n_k0 = cfg.R ** 3 * cfg.feat_dim
+    adam_bytes = 32 * n_k0
+    achieved = adam_bytes / (adam_ms * 1e-3) / 1e9
+    counts = trainer.last_counts
+    sb = step_bytes(cfg, counts)
+
+    res = {
+        "metric": "fine train-step rays/s at 256^3",
+        "value": cfg.n_rays / (ms * 1e-3),
+        "unit": "rays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "p50_ms": per[len(per) // 2],
+        "p99_ms": per[min(len(per) - 1, int(round(0.99 * (len(per) - 1))))],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.mlp == "fp32" else "f32 (rgbnet bf16)",
+        "data": "synthetic (seeded analytic SDF scene, random-init features and rgbnet)",
+        "config": {"workload": "plain fine step: 256^3 SDF + 256^3x12 features, rgbnet "
+                               "42-192x3-3, 8192 rays, stepsize 0.5, s=256, fwd+bwd+Adam",
+                   "rays_per_step": cfg.n_rays, "path": args.path, "mlp": args.mlp},
+        "roofline": {"bound": "hbm", "kernel": "adam_step (k0 grid, 201M fp32 elements)",
+                     "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": None},
+        "step_roofline": {"algorithmic_bytes": sb, "achieved_GBs": sb / (ms * 1e-3) / 1e9,
+                          "frac": sb / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS},
+        "samples": counts,
+        "wall_s": wall,
+    }
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        res["cpu_baseline"] = cpu_baseline(cfg, threads)
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -117,6 +117,147 @@ int vx_alpha2weight_backward(int dtype, const void* alpha, const void* weight, c
                              const void* grad_weights, const void* grad_last, void* grad,
                              void* stream);
 
+/* ---- dense grids --------------------------------------------------------------- */
+/* A grid is described by geom[8] = {C, X, Y, Z, stride_c, stride_x, stride_y, stride_z}
+ * (element strides): the reference's contiguous [1,C,X,Y,Z] tensor and the
+ * channels-last layout are both accepted.  xyz_min/xyz_max are [3] device arrays. */
+
+/* lib/grid.py:47-58 DenseGrid.forward (F.grid_sample, bilinear, align_corners=True,
+ * zero padding): out [n, C]. */
+int vx_grid_sample_fwd(int dtype, const void* grid, const int64_t* geom, const void* xyz,
+                       const void* xyz_min, const void* xyz_max, int64_t n, void* out,
+                       void* stream);
+/* Adjoint of vx_grid_sample_fwd w.r.t. the grid (the grid_sample backward autograd
+ * runs for lib/grid.py:54): grad += scatter of gout [n, C] (atomic adds). */
+int vx_grid_sample_bwd(int dtype, void* grad, const int64_t* geom, const void* xyz,
+                       const void* xyz_min, const void* xyz_max, int64_t n, const void* gout,
+                       void* stream);
+
+/* lib/voxurf_fine.py:537-577 Voxurf.sample_sdfs: feat [n, 6*nd], grad [n, 3*nd] (zyx
+ * component order, as the reference), disp [nd] device array of displacements. */
+int vx_sample_sdfs_fwd(int dtype, const void* grid, const int64_t* geom, const void* xyz,
+                       const void* xyz_min, const void* xyz_max, const void* disp, int nd,
+                       double voxel_size, int use_grad_norm, int64_t n, void* feat, void* grad,
+                       void* stream);
+/* Adjoint of vx_sample_sdfs_fwd: ggrid += d(feat, grad)/d(grid)^T (gfeat, ggrad);
+ * either gradient may be NULL. */
+int vx_sample_sdfs_bwd(int dtype, void* ggrid, const void* grid, const int64_t* geom,
+                       const void* xyz, const void* xyz_min, const void* xyz_max,
+                       const void* disp, int nd, double voxel_size, int use_grad_norm, int64_t n,
+                       const void* gfeat, const void* ggrad, void* stream);
+
+/* lib/voxurf_fine.py:502-534 Voxurf.grid_sampler(xyz, sdf_grid, sample_ret=True,
+ * sample_grad=True): sdf [n], gradient [n,3] (xyz order), feat [n,6] (may be NULL). */
+int vx_sdf_grad_fwd(int dtype, const void* grid, const int64_t* geom, const void* xyz,
+                    const void* xyz_min, const void* xyz_max, double voxel_size, int64_t n,
+                    void* sdf, void* grad, void* feat, void* stream);
+/* Adjoint of vx_sdf_grad_fwd; gsdf / ggrad / gfeat may be NULL. */
+int vx_sdf_grad_bwd(int dtype, void* ggrid, const int64_t* geom, const void* xyz,
+                    const void* xyz_min, const void* xyz_max, double voxel_size, int64_t n,
+                    const void* gsdf, const void* ggrad, const void* gfeat, void* stream);
+
+/* lib/voxurf_fine.py:440-450 neus_sdf_gradient(mode='interpolate'): sdf [X,Y,Z] ->
+ * out [3,X,Y,Z]; and its adjoint (gsdf += ...). */
+int vx_dense_sdf_gradient_fwd(int dtype, const void* sdf, int X, int Y, int Z, double voxel_size,
+                              void* out, void* stream);
+int vx_dense_sdf_gradient_bwd(int dtype, const void* gout, int X, int Y, int Z,
+                              double voxel_size, void* gsdf, void* stream);
+
+/* lib/cuda/total_variation.cpp:16-20 total_variation_add_grad
+ * (total_variation_kernel.cu:13-35, 68-98): grad += TV gradient, in place. */
+int vx_total_variation_add_grad(int dtype, const void* param, void* grad, const int64_t* geom,
+                                float wx, float wy, float wz, int dense_mode, void* stream);
+/* lib/cuda/total_variation.cpp:22-27 total_variation_add_grad_new
+ * (total_variation_kernel.cu:38-66, 101-133); mask has the grid's layout. */
+int vx_total_variation_add_grad_new(int dtype, const void* param, void* grad, const void* mask,
+                                    const int64_t* geom, float wx, float wy, float wz,
+                                    int dense_mode, void* stream);
+
+/* Zero n elements (a fill kernel, safe inside captured graphs). */
+int vx_zero(int dtype, void* p, int64_t n, void* stream);
+
+/* ---- optimisers ---------------------------------------------------------------- */
+
+/* lib/cuda/adam_upd.cpp:36-86 adam_upd / masked_adam_upd / adam_upd_with_perlr
+ * (adam_upd_kernel.cu:8-132): masked != 0 skips grad == 0; perlr may be NULL. */
+int vx_adam_upd(int dtype, void* param, const void* grad, void* exp_avg, void* exp_avg_sq,
+                const void* perlr, int64_t n, int step, float beta1, float beta2, float lr,
+                float eps, int masked, void* stream);
+
+/* lib/utils.py:154-199 adam() for one parameter (the optimiser the training step
+ * runs): step is the 1-based step count after increment; per_lr may be NULL;
+ * clear_grad != 0 zeroes grad after use. */
+int vx_adam_step(int dtype, void* param, void* grad, void* exp_avg, void* exp_avg_sq,
+                 const void* per_lr, int64_t n, int step, double lr, double beta1, double beta2,
+                 double eps, int clear_grad, void* stream);
+
+/* ---- per-sample ops ------------------------------------------------------------ */
+
+/* lib/voxurf_fine.py:463-500 neus_alpha_from_sdf_scatter (use_mid=True): alpha [n]. */
+int vx_neus_alpha_fwd(int dtype, const void* viewdirs, const int64_t* ray_id, const void* sdf,
+                      const void* gradient, int64_t n, double dist, double inv_s, void* alpha,
+                      void* stream);
+/* Its adjoint: gsdf [n], ggradient [n,3]. */
+int vx_neus_alpha_bwd(int dtype, const void* viewdirs, const int64_t* ray_id, const void* sdf,
+                      const void* gradient, int64_t n, double dist, double inv_s,
+                      const void* galpha, void* gsdf, void* ggradient, void* stream);
+
+/* torch_scatter.segment_coo(src, index, out, reduce='sum') as called at
+ * lib/voxurf_fine.py:753-777: out [n_out, C] += segment sums of src [n, C] over the
+ * sorted index [n]. */
+int vx_segment_sum(int dtype, const void* src, const int64_t* index, int64_t n, int C,
+                   int64_t n_out, void* out, void* stream);
+/* Its adjoint: gsrc [n, C] = gout[index]. */
+int vx_segment_gather(int dtype, const void* gout, const int64_t* index, int64_t n, int C,
+                      int64_t n_out, void* gsrc, void* stream);
+
+/* ---- fused fine step (per-ray kernels) ------------------------------------------ */
+/* These fuse the per-sample stages of lib/voxurf_fine.py:620-802 (sample_ray,
+ * grid_sampler with sample_grad, neus_alpha_from_sdf_scatter, the two fast_color_thres
+ * compactions and Alphas2Weights) into one pass per ray, and their adjoints
+ * (Alphas2Weights.backward -> render_utils_kernel.cu:654-677, the NeuS and grid_sample
+ * adjoints that autograd runs) into a per-ray recomputing backward.  Host-side arrays:
+ * sdf_geom/k0_geom as for the grid ops; box[6] = {xyz_min, xyz_max};
+ * params[6] = {stepdist, voxel_size, inv_s, fast_color_thres, near, far}.
+ * Per-ray survivor slots hold `cap` entries of two float4 each (s4: x, y, z, weight;
+ * g4: gradient xyz, step). */
+int vx_fused_fine_fwd(const float* sdf, const int64_t* sdf_geom, const float* box,
+                      const float* params, const float* rays_o, const float* rays_d,
+                      const float* viewdirs, int64_t n_rays, int cap, void* s4, void* g4,
+                      int64_t* cnt, float* last, int* stats, int count_all, int* overflow,
+                      void* stream);
+/* Compacts survivors to rows off_incl[r]-cnt[r] .. off_incl[r]-1 and builds the rgbnet
+ * input rows X [M, xstride] = [k0 | normal | view PE | 0] (lib/voxurf_fine.py:678-716);
+ * P4/G4 [M] float4 copies of s4/g4, rid [M] int32 ray ids. */
+int vx_fused_fine_gather(const float* k0, const int64_t* k0_geom, const int64_t* sdf_geom,
+                         const float* box, const float* params, const float* viewdirs,
+                         int64_t n_rays, int cap, int n_freq, const void* s4, const void* g4,
+                         const int64_t* cnt, const int64_t* off_incl, int xstride, float* X,
+                         void* P4, void* G4, int* rid, void* stream);
+/* segment_coo compositing (lib/voxurf_fine.py:753-763), MSE loss per ray (run.py:604)
+ * and the colour-side adjoints: dlogit [M,3], gw [M] (d loss / d weight), glast [N],
+ * S [N] (sum of gw*w per ray), loss_r [N] (sum of squared errors). */
+int vx_fused_fine_composite(int64_t n_rays, const int64_t* cnt, const int64_t* off_incl,
+                            const float* logits, int lstride, const void* P4, const float* last,
+                            const float* target, float bg, float* rgb, float* dlogit, float* gw,
+                            float* glast, float* S, float* loss_r, void* stream);
+/* Adjoint of the feature rows: k0_grad += trilinear scatter of dX[:, :C]; dgf [M,3] =
+ * d loss / d gradient through the normalised normal dX[:, C:C+3]. */
+int vx_fused_fine_feat_bwd(const int64_t* k0_geom, const int64_t* sdf_geom, const float* box,
+                           const float* params, int64_t m, const float* dX, int dxs,
+                           const void* P4, const void* G4, float* k0_grad, float* dgf,
+                           void* stream);
+/* Per-ray recomputing backward of the SDF side: sdf_grad += adjoint of (7-point SDF
+ * lookup -> NeuS alpha -> transmittance walk) for every sample of the walk range. */
+int vx_fused_fine_bwd(const float* sdf, const int64_t* sdf_geom, const float* box,
+                      const float* params, const float* rays_o, const float* rays_d,
+                      const float* viewdirs, int64_t n_rays, const int64_t* cnt,
+                      const int64_t* off_incl, const float* last, const float* gw,
+                      const float* glast, const float* S, const float* dgf, float* sdf_grad,
+                      void* stream);
+/* *out = sum of x[0..n) (float32, device scalar). */
+int vx_sum_f32(const float* x, int64_t n, float* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

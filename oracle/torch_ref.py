@@ -1,0 +1,303 @@
+"""PyTorch-on-CPU restatement of Voxurf's Python-level hot path -- TEST
+INFRASTRUCTURE ONLY (the checker for voxurf_amd, and the CPU baseline bench.py times).
+
+Each function restates the cited reference code with the same torch operations in the
+same order, on CPU tensors (float32 or float64).  The native ops (ray sampling,
+alpha2weight and its adjoint) come from the sequential C restatement in oracle.c.
+Third-party pieces are restated from their definitions: torch_scatter.segment_coo
+(reduce='sum', sorted index) as an index_add_ over the rows in order.
+"""
+import math
+
+import numpy as np
+import torch
+import torch.nn.functional as F
+
+import oracle
+
+
+# ---- grid lookups ---------------------------------------------------------------------
+
+def dense_grid_forward(grid, xyz, xyz_min, xyz_max):
+    """lib/grid.py:47-58."""
+    C = grid.shape[1]
+    shape = xyz.shape[:-1]
+    xyz = xyz.reshape(1, 1, 1, -1, 3)
+    ind_norm = ((xyz - xyz_min) / (xyz_max - xyz_min)).flip((-1,)) * 2 - 1
+    out = F.grid_sample(grid, ind_norm, mode="bilinear", align_corners=True)
+    out = out.reshape(C, -1).T.reshape(*shape, C)
+    if C == 1:
+        out = out.squeeze(-1)
+    return out
+
+
+def sample_sdfs(xyz, grid, displace_list, xyz_min, xyz_max, voxel_size, use_grad_norm=False):
+    """lib/voxurf_fine.py:537-577."""
+    shape = xyz.shape[:-1]
+    xyz = xyz.reshape(1, 1, 1, -1, 3)
+    ind_norm = ((xyz - xyz_min) / (xyz_max - xyz_min)).flip((-1,)) * 2 - 1
+    gs = grid.size()[-3:]
+    size_zyx = torch.tensor([gs[2], gs[1], gs[0]])
+    ind = ((ind_norm + 1) / 2) * (size_zyx - 1)
+    offset = torch.tensor([[-1, 0, 0], [1, 0, 0], [0, -1, 0], [0, 1, 0], [0, 0, -1], [0, 0, 1]])
+    displace = torch.tensor(displace_list, dtype=grid.dtype)
+    offset = offset[:, None, :] * displace[None, :, None]
+    all_ind = ind.unsqueeze(-2) + offset.view(-1, 3)
+    all_ind = all_ind.view(1, 1, 1, -1, 3)
+    all_ind[..., 0] = all_ind[..., 0].clamp(min=0, max=size_zyx[0] - 1)
+    all_ind[..., 1] = all_ind[..., 1].clamp(min=0, max=size_zyx[1] - 1)
+    all_ind[..., 2] = all_ind[..., 2].clamp(min=0, max=size_zyx[2] - 1)
+    all_ind_norm = (all_ind / (size_zyx - 1)) * 2 - 1
+    feat = F.grid_sample(grid, all_ind_norm, mode="bilinear", align_corners=True)
+    nd = len(displace_list)
+    all_ind = all_ind.view(1, 1, 1, -1, 6, nd, 3)
+    diff = all_ind[:, :, :, :, 1::2, :, :] - all_ind[:, :, :, :, 0::2, :, :]
+    diff, _ = diff.max(dim=-1)
+    feat_ = feat.view(1, 1, 1, -1, 6, nd)
+    feat_diff = feat_[:, :, :, :, 1::2, :] - feat_[:, :, :, :, 0::2, :]
+    grad = feat_diff / diff / voxel_size
+    feat = feat.view(shape[-1], 6, nd)
+    grad = grad.view(shape[-1], 3, nd)
+    if use_grad_norm:
+        grad = grad / (grad.norm(dim=1, keepdim=True) + 1e-5)
+    return feat.view(shape[-1], 6 * nd), grad.view(shape[-1], 3 * nd)
+
+
+def grid_sampler(xyz, grid, xyz_min, xyz_max, voxel_size):
+    """lib/voxurf_fine.py:502-534 with sample_ret=True, sample_grad=True -> sdf, grad, feat."""
+    shape = xyz.shape[:-1]
+    sdf = dense_grid_forward(grid, xyz, xyz_min, xyz_max).reshape(*shape)
+    feat, grad = sample_sdfs(xyz, grid, [1.0], xyz_min, xyz_max, voxel_size, use_grad_norm=False)
+    feat = torch.cat([feat[:, 4:6], feat[:, 2:4], feat[:, 0:2]], dim=-1)
+    grad = torch.cat([grad[:, [2]], grad[:, [1]], grad[:, [0]]], dim=-1)
+    return sdf, grad, feat
+
+
+def neus_sdf_gradient(sdf, voxel_size):
+    """lib/voxurf_fine.py:440-450 ('interpolate')."""
+    gradient = torch.zeros([1, 3] + [*sdf.shape[-3:]], dtype=sdf.dtype)
+    gradient[:, 0, 1:-1, :, :] = (sdf[:, 0, 2:, :, :] - sdf[:, 0, :-2, :, :]) / 2 / voxel_size
+    gradient[:, 1, :, 1:-1, :] = (sdf[:, 0, :, 2:, :] - sdf[:, 0, :, :-2, :]) / 2 / voxel_size
+    gradient[:, 2, :, :, 1:-1] = (sdf[:, 0, :, :, 2:] - sdf[:, 0, :, :, :-2]) / 2 / voxel_size
+    return gradient
+
+
+def neus_alpha(viewdirs, ray_id, dist, sdf, gradients, inv_s):
+    """lib/voxurf_fine.py:475-500 (use_mid=True, cos_anneal_ratio=1)."""
+    dirs = viewdirs[ray_id]
+    inv_s = torch.as_tensor(inv_s, dtype=sdf.dtype).reshape(1)
+    dist = torch.as_tensor(dist, dtype=sdf.dtype)
+    true_cos = (dirs * gradients).sum(-1, keepdim=True)
+    cos_anneal_ratio = 1.0
+    iter_cos = -(F.relu(-true_cos * 0.5 + 0.5) * (1.0 - cos_anneal_ratio) +
+                 F.relu(-true_cos) * cos_anneal_ratio)
+    sdf = sdf.unsqueeze(-1)
+    estimated_next_sdf = sdf + iter_cos * dist.reshape(-1, 1) * 0.5
+    estimated_prev_sdf = sdf - iter_cos * dist.reshape(-1, 1) * 0.5
+    prev_cdf = torch.sigmoid(estimated_prev_sdf * inv_s.reshape(-1, 1))
+    next_cdf = torch.sigmoid(estimated_next_sdf * inv_s.reshape(-1, 1))
+    p = prev_cdf - next_cdf
+    c = prev_cdf
+    return ((p + 1e-5) / (c + 1e-5)).clip(0.0, 1.0).squeeze(-1)
+
+
+class Alphas2Weights(torch.autograd.Function):
+    """lib/voxurf_fine.py:981-997 on the C oracle's alpha2weight (+ adjoint)."""
+
+    @staticmethod
+    def forward(ctx, alpha, ray_id, N):
+        a = alpha.detach().numpy()
+        w, T, last, i_s, i_e = oracle.alpha2weight(a, ray_id.numpy(), N)
+        ctx.saved = (a, w, T, last, i_s, i_e)
+        ctx.N = N
+        return torch.from_numpy(w), torch.from_numpy(last)
+
+    @staticmethod
+    def backward(ctx, gw, gl):
+        a, w, T, last, i_s, i_e = ctx.saved
+        g = oracle.alpha2weight_backward(a, w, T, last, i_s, i_e, ctx.N,
+                                         gw.contiguous().numpy().astype(a.dtype),
+                                         gl.contiguous().numpy().astype(a.dtype))
+        return torch.from_numpy(g), None, None
+
+
+def segment_sum(src, index, N):
+    """torch_scatter.segment_coo(src, index, out=zeros([N, ...]), reduce='sum')."""
+    out = torch.zeros((N,) + tuple(src.shape[1:]), dtype=src.dtype)
+    return out.index_add(0, index, src)
+
+
+def view_pe(viewdirs, n_freq):
+    """lib/voxurf_fine.py:694-695: [v, sin(v (x) F), cos(v (x) F)], F = 2^0..2^(n-1)."""
+    freq = torch.tensor([2.0 ** i for i in range(n_freq)], dtype=viewdirs.dtype)
+    emb = (viewdirs.unsqueeze(-1) * freq).flatten(-2)
+    return torch.cat([viewdirs, emb.sin(), emb.cos()], -1)
+
+
+def mlp(x, params):
+    """rgbnet: Linear+ReLU ... Linear (lib/voxurf_fine.py:132-139)."""
+    h = x
+    for i, (W, b) in enumerate(params):
+        h = F.linear(h, W, b)
+        if i < len(params) - 1:
+            h = F.relu(h)
+    return h
+
+
+# ---- total variation and Adam -----------------------------------------------------------
+
+def tv_add_grad(param, grad, wx, wy, wz, dense_mode, mask=None):
+    """total_variation_kernel.cu:13-66 (in place on grad), closed form."""
+    wx, wy, wz = wx / 6, wy / 6, wz / 6
+    p = param[0]
+    add = torch.zeros_like(p)
+    if mask is None:
+        wk, wj, wi = wz, wy, wz
+        m = None
+    else:
+        wk, wj, wi = wx, wy, wz
+        m = mask[0]
+    for axis, w in ((3, wk), (2, wj), (1, wi)):
+        d = torch.diff(p, dim=axis).clamp(-1, 1)          # p[i+1] - p[i]
+        t_lo = w * (-d)                                     # term at i:   p[i] - p[i+1]
+        t_hi = w * d                                        # term at i+1: p[i+1] - p[i]
+        if m is not None:
+            mm = m.narrow(axis, 0, m.shape[axis] - 1) * m.narrow(axis, 1, m.shape[axis] - 1)
+            t_lo = t_lo * mm
+            t_hi = t_hi * mm
+        add.narrow(axis, 0, p.shape[axis] - 1).add_(t_lo)
+        add.narrow(axis, 1, p.shape[axis] - 1).add_(t_hi)
+    if not dense_mode:
+        add = torch.where(grad[0] != 0, add, torch.zeros_like(add))
+    grad[0] += add
+
+
+def adam_step(param, grad, exp_avg, exp_avg_sq, step, lr, beta1, beta2, eps, per_lr=None):
+    """lib/utils.py:169-199 for one parameter (in place)."""
+    bias_correction1 = 1 - beta1 ** step
+    bias_correction2 = 1 - beta2 ** step
+    exp_avg.mul_(beta1).add_(grad, alpha=1 - beta1)
+    exp_avg_sq.mul_(beta2).addcmul_(grad, grad, value=1 - beta2)
+    denom = (exp_avg_sq.sqrt() / math.sqrt(bias_correction2)).add_(eps)
+    step_size = lr / bias_correction1
+    if per_lr is not None:
+        param.addcdiv_(exp_avg * per_lr, denom, value=-step_size)
+    else:
+        param.addcdiv_(exp_avg, denom, value=-step_size)
+
+
+def adam_upd(param, grad, exp_avg, exp_avg_sq, step, beta1, beta2, lr, eps, masked=False,
+             perlr=None):
+    """adam_upd_kernel.cu:8-58, 72 (in place)."""
+    step_size = np.float32(lr) * np.sqrt(np.float32(1) - np.float32(beta2) ** np.float32(step)) / (
+        np.float32(1) - np.float32(beta1) ** np.float32(step))
+    sel = (grad != 0) if masked else torch.ones_like(grad, dtype=torch.bool)
+    m = beta1 * exp_avg + (1 - beta1) * grad
+    v = beta2 * exp_avg_sq + (1 - beta2) * grad * grad
+    ss = float(step_size) * (perlr if perlr is not None else 1.0)
+    upd = param - ss * m / (v.sqrt() + eps)
+    exp_avg.copy_(torch.where(sel, m, exp_avg))
+    exp_avg_sq.copy_(torch.where(sel, v, exp_avg_sq))
+    param.copy_(torch.where(sel, upd, param))
+
+
+# ---- the training steps -------------------------------------------------------------------
+
+def sample_ray(rays_o, rays_d, xyz_min, xyz_max, stepdist):
+    """lib/voxurf_fine.py:593-617 on the C oracle sampler (far forced to 1e9)."""
+    pts, mask, rid, sid, ns, tmin, tmax = oracle.sample_pts_on_rays(
+        rays_o.numpy(), rays_d.numpy(), xyz_min.numpy(), xyz_max.numpy(), 0.0, 1e9,
+        np.float32(stepdist))
+    inb = ~mask
+    return (torch.from_numpy(pts[inb]), torch.from_numpy(rid[inb]), torch.from_numpy(sid[inb]),
+            torch.from_numpy(mask))
+
+
+def fine_forward(state, rays_o, rays_d, viewdirs, cfg):
+    """Fine-stage forward of the plain fine step (lib/voxurf_fine.py:620-802 with the
+    rgbnet input k0 (12) | normal (3) | view PE (27)); returns (rgb_marched, extras)."""
+    N = rays_o.shape[0]
+    xyz_min, xyz_max = state["xyz_min"], state["xyz_max"]
+    ray_pts, ray_id, step_id, _ = sample_ray(rays_o, rays_d, xyz_min, xyz_max, cfg.stepdist)
+    sdf, gradient, _ = grid_sampler(ray_pts, state["sdf"], xyz_min, xyz_max, cfg.voxel_size)
+    alpha = neus_alpha(viewdirs, ray_id, cfg.stepdist, sdf, gradient, cfg.inv_s)
+    m = alpha > cfg.fast_color_thres
+    alpha, ray_id, ray_pts, gradient, sdf = alpha[m], ray_id[m], ray_pts[m], gradient[m], sdf[m]
+    weights, alphainv_last = Alphas2Weights.apply(alpha, ray_id, N)
+    m = weights > cfg.fast_color_thres
+    weights, ray_id, ray_pts, gradient = weights[m], ray_id[m], ray_pts[m], gradient[m]
+    k0 = dense_grid_forward(state["k0"], ray_pts, xyz_min, xyz_max)
+    normal = gradient / (gradient.norm(dim=-1, keepdim=True) + 1e-5)
+    vemb = view_pe(viewdirs, cfg.view_pe)
+    rgb_feat = torch.cat([k0, normal, vemb[ray_id]], -1)
+    rgb = torch.sigmoid(mlp(rgb_feat, state["mlp"]))
+    rgb_marched = segment_sum(weights.unsqueeze(-1) * rgb, ray_id, N) + \
+        alphainv_last[..., None] * cfg.bg
+    return rgb_marched, dict(n_samples=len(ray_id), weights=weights, ray_id=ray_id,
+                             alphainv_last=alphainv_last)
+
+
+def coarse_forward(state, rays_o, rays_d, viewdirs, cfg):
+    """Coarse-stage forward of the plain coarse step (lib/voxurf_coarse.py:513-619 with
+    6-neighbour normals and colour = sigmoid(k0))."""
+    N = rays_o.shape[0]
+    xyz_min, xyz_max = state["xyz_min"], state["xyz_max"]
+    ray_pts, ray_id, step_id, _ = sample_ray(rays_o, rays_d, xyz_min, xyz_max, cfg.stepdist)
+    sdf, gradient, _ = grid_sampler(ray_pts, state["sdf"], xyz_min, xyz_max, cfg.voxel_size)
+    alpha = neus_alpha(viewdirs, ray_id, cfg.stepdist, sdf, gradient, cfg.inv_s)
+    weights, alphainv_last = Alphas2Weights.apply(alpha, ray_id, N)
+    m = weights > cfg.fast_color_thres
+    ray_pts, ray_id, alpha = ray_pts[m], ray_id[m], alpha[m]
+    weights, alphainv_last = Alphas2Weights.apply(alpha, ray_id, N)
+    rgb = torch.sigmoid(dense_grid_forward(state["k0"], ray_pts, xyz_min, xyz_max))
+    rgb_marched = segment_sum(weights.unsqueeze(-1) * rgb, ray_id, N)
+    cum_w = segment_sum(weights.unsqueeze(-1), ray_id, N)
+    rgb_marched = (rgb_marched + (1 - cum_w) * cfg.bg).clamp(0, 1)
+    return rgb_marched, dict(n_samples=len(ray_id), weights=weights, ray_id=ray_id,
+                             alphainv_last=alphainv_last)
+
+
+def make_state(cfg, dtype=torch.float32, seed=None):
+    from voxurf_amd import scene
+    seed = cfg.seed if seed is None else seed
+    st = dict(
+        xyz_min=torch.tensor([-1.0, -1.0, -1.0], dtype=dtype),
+        xyz_max=torch.tensor([1.0, 1.0, 1.0], dtype=dtype),
+        sdf=scene.sdf_grid(cfg.R, seed, dtype),
+        k0=scene.feature_grid(cfg.R, cfg.feat_dim, seed, dtype),
+        mlp=scene.mlp_params(cfg.mlp_in, cfg.mlp_width, cfg.mlp_depth, seed, dtype)
+        if cfg.mlp_depth > 0 else [],
+    )
+    return st
+
+
+def train_step(state, opt_state, rays_o, rays_d, viewdirs, target, cfg, step, coarse=False):
+    """One training step (run.py:600-659): forward, MSE loss, backward, TV (coarse),
+    Adam (utils.Adam, betas (0.9, 0.99)).  Mutates state/opt_state; returns a dict."""
+    params = [("sdf", state["sdf"], cfg.lr_sdf), ("k0", state["k0"], cfg.lr_k0)]
+    for i, (W, b) in enumerate(state["mlp"]):
+        params += [(f"mlp{i}.w", W, cfg.lr_mlp), (f"mlp{i}.b", b, cfg.lr_mlp)]
+    for _, p, _ in params:
+        p.requires_grad_(True)
+        p.grad = None
+    fwd = coarse_forward if coarse else fine_forward
+    rgb, extra = fwd(state, rays_o, rays_d, viewdirs, cfg)
+    loss = F.mse_loss(rgb, target)
+    loss.backward()
+    grads = {}
+    with torch.no_grad():
+        if cfg.tv_weight > 0:
+            tv_add_grad(state["sdf"], state["sdf"].grad, cfg.tv_weight, cfg.tv_weight,
+                        cfg.tv_weight, True)
+            tv_add_grad(state["k0"], state["k0"].grad, cfg.tv_weight, cfg.tv_weight,
+                        cfg.tv_weight, True)
+        for name, p, lr in params:
+            g = p.grad if p.grad is not None else torch.zeros_like(p)
+            grads[name] = g.clone()
+            st = opt_state.setdefault(name, dict(step=0, m=torch.zeros_like(p), v=torch.zeros_like(p)))
+            st["step"] += 1
+            adam_step(p, g, st["m"], st["v"], st["step"], lr, 0.9, 0.99, 1e-8)
+    for _, p, _ in params:
+        p.requires_grad_(False)
+        p.grad = None
+    return dict(loss=float(loss), rgb=rgb.detach(), grads=grads, **extra)

@@ -146,7 +146,7 @@ def test_alpha2weight_bitexact(dev, dtype):
         _eq(a, b)
     gw = rng.normal(size=len(rid)).astype(dtype)
     gl = rng.normal(size=4100).astype(dtype)
-    g = ru.alpha2weight_backward(*out, 4100, _t(gw, dev), _t(gl, dev))
+    g = ru.alpha2weight_backward(_t(alpha, dev), *out, 4100, _t(gw, dev), _t(gl, dev))
     rg = oracle.alpha2weight_backward(alpha, *ref, 4100, gw, gl)
     g = g.cpu().numpy()
     scale = np.abs(rg).max()

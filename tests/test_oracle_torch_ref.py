@@ -1,0 +1,172 @@
+"""Pins of oracle/torch_ref.py, the CPU restatement of the Python-level hot path.
+
+Each restated function is checked against something the mathematics fixes: trilinear
+interpolation and central differences are exact on linear fields; the TV gradient is
+the autograd gradient of a smooth-L1 energy; Adam at t=1 has a closed form; the segment
+sum against a Python loop; NeuS alpha special cases; a planted mistake in a restatement
+must fail these checks.
+"""
+import numpy as np
+import pytest
+import torch
+import torch.nn.functional as F
+
+from oracle import torch_ref as tr
+
+MN = torch.tensor([-1.0, -1.0, -1.0], dtype=torch.float64)
+MX = torch.tensor([1.0, 1.0, 1.0], dtype=torch.float64)
+
+
+def lattice(R):
+    x = torch.linspace(-1, 1, R, dtype=torch.float64)
+    return torch.meshgrid(x, x, x, indexing="ij")
+
+
+def linear_grid(R, a=(0.3, -0.7, 1.1), b=0.2):
+    X, Y, Z = lattice(R)
+    return (a[0] * X + a[1] * Y + a[2] * Z + b)[None, None]
+
+
+def test_dense_grid_exact_on_linear_field():
+    g = linear_grid(9)
+    pts = torch.rand(500, 3, dtype=torch.float64) * 2 - 1
+    out = tr.dense_grid_forward(g, pts, MN, MX)
+    exp = 0.3 * pts[:, 0] - 0.7 * pts[:, 1] + 1.1 * pts[:, 2] + 0.2
+    torch.testing.assert_close(out, exp, rtol=0, atol=1e-12)
+    # lattice points return the stored voxel value exactly (multi-channel, axis order)
+    G = torch.randn(1, 4, 5, 6, 7, dtype=torch.float64)
+    i, j, k = 2, 3, 4
+    p = torch.tensor([[-1 + 2 * i / 4, -1 + 2 * j / 5, -1 + 2 * k / 6]], dtype=torch.float64)
+    torch.testing.assert_close(tr.dense_grid_forward(G, p, MN, MX)[0], G[0, :, i, j, k])
+
+
+def test_six_neighbour_gradient_exact_on_linear_field():
+    R = 11
+    g = linear_grid(R)
+    vs = 2 / (R - 1)
+    pts = torch.rand(400, 3, dtype=torch.float64) * 2 - 1
+    pts[:5] = torch.tensor([[-1, -1, -1], [1, 1, 1], [-1, 0.3, 1], [0.999, -0.999, 0], [0, 0, 0]])
+    sdf, grad, feat = tr.grid_sampler(pts, g, MN, MX, vs)
+    # one-sided at clamped borders, still exact for a linear field (denominator shrinks)
+    torch.testing.assert_close(grad, torch.tensor([0.3, -0.7, 1.1], dtype=torch.float64).expand_as(grad),
+                               rtol=0, atol=1e-10)
+    # feat order (-x,+x,-y,+y,-z,+z): interior point neighbours are +-1 voxel
+    p = torch.tensor([[0.1, 0.2, -0.3]], dtype=torch.float64)
+    _, _, f = tr.grid_sampler(p, g, MN, MX, vs)
+    base = 0.3 * 0.1 - 0.7 * 0.2 + 1.1 * -0.3 + 0.2
+    exp = [base - 0.3 * vs, base + 0.3 * vs, base + 0.7 * vs, base - 0.7 * vs, base - 1.1 * vs, base + 1.1 * vs]
+    torch.testing.assert_close(f[0], torch.tensor(exp, dtype=torch.float64), rtol=0, atol=1e-12)
+    # hierarchical displacements with normalisation give the unit direction
+    feat, gr = tr.sample_sdfs(pts[5:], g, [0.5, 1.0, 2.0], MN, MX, vs, use_grad_norm=True)
+    a = torch.tensor([1.1, -0.7, 0.3], dtype=torch.float64)  # zyx
+    unit = a / (a.norm() + 1e-5)
+    torch.testing.assert_close(gr.view(-1, 3, 3), unit[None, :, None].expand(len(pts) - 5, 3, 3),
+                               rtol=0, atol=1e-9)
+
+
+def test_dense_gradient_closed_form():
+    g = linear_grid(6)
+    vs = 2 / 5
+    out = tr.neus_sdf_gradient(g, vs)
+    assert torch.allclose(out[0, 0, 1:-1], torch.tensor(0.3, dtype=torch.float64))
+    assert torch.allclose(out[0, 2, :, :, 1:-1], torch.tensor(1.1, dtype=torch.float64))
+    assert torch.all(out[0, 0, 0] == 0) and torch.all(out[0, 0, -1] == 0)
+    assert torch.all(out[0, 1, :, 0] == 0) and torch.all(out[0, 2, :, :, -1] == 0)
+
+
+def test_neus_alpha_special_cases_and_autograd():
+    n = 64
+    v = torch.randn(4, 3, dtype=torch.float64)
+    v = v / v.norm(dim=-1, keepdim=True)
+    rid = torch.randint(0, 4, (n,))
+    sdf = torch.randn(n, dtype=torch.float64) * 0.05
+    grad = torch.randn(n, 3, dtype=torch.float64)
+    # cos >= 0  =>  ic = 0  =>  alpha = 1e-5 / (sigmoid(sdf*s) + 1e-5)
+    g_pos = v[rid] * 2.0
+    a = tr.neus_alpha(v, rid, 0.01, sdf, g_pos, 64.0)
+    c = torch.sigmoid(sdf * 64.0)
+    torch.testing.assert_close(a, 1e-5 / (c + 1e-5))
+    # general case against the closed form in float64
+    a = tr.neus_alpha(v, rid, 0.01, sdf, grad, 64.0)
+    cos = (v[rid] * grad).sum(-1)
+    ic = -torch.clamp(-cos, min=0)
+    prev = torch.sigmoid((sdf - ic * 0.005) * 64)
+    nxt = torch.sigmoid((sdf + ic * 0.005) * 64)
+    torch.testing.assert_close(a, ((prev - nxt + 1e-5) / (prev + 1e-5)).clamp(0, 1))
+
+
+def test_tv_equals_smooth_l1_autograd():
+    torch.manual_seed(0)
+    p = torch.randn(1, 2, 5, 6, 7, dtype=torch.float64) * 1.5
+    for wx, wy, wz in ((1.0, 1.0, 1.0), (0.3, 0.5, 0.7)):
+        g = torch.zeros_like(p)
+        tr.tv_add_grad(p, g, wx, wy, wz, dense_mode=True)
+        q = p.clone().requires_grad_(True)
+        # reference weighting: k (last) axis and i axis take wz, j axis wy (wx unused)
+        e = (wz / 6 * F.smooth_l1_loss(q[..., 1:], q[..., :-1], reduction="sum", beta=1.0) +
+             wy / 6 * F.smooth_l1_loss(q[:, :, :, 1:], q[:, :, :, :-1], reduction="sum", beta=1.0) +
+             wz / 6 * F.smooth_l1_loss(q[:, :, 1:], q[:, :, :-1], reduction="sum", beta=1.0))
+        e.backward()
+        torch.testing.assert_close(g, q.grad)
+    # masked variant: k axis takes wx; pairs count only where both masks are 1
+    mask = (torch.rand_like(p) > 0.3).double()
+    g = torch.zeros_like(p)
+    tr.tv_add_grad(p, g, 0.3, 0.5, 0.7, True, mask=mask)
+    q = p.clone().requires_grad_(True)
+    def term(axis, w):
+        a, b = q.narrow(axis, 1, q.shape[axis] - 1), q.narrow(axis, 0, q.shape[axis] - 1)
+        m = mask.narrow(axis, 1, q.shape[axis] - 1) * mask.narrow(axis, 0, q.shape[axis] - 1)
+        return w / 6 * (F.smooth_l1_loss(a, b, reduction="none", beta=1.0) * m).sum()
+    (term(4, 0.3) + term(3, 0.5) + term(2, 0.7)).backward()
+    torch.testing.assert_close(g, q.grad)
+    # sparse mode: voxels with zero grad are left untouched
+    g = torch.zeros_like(p)
+    g[0, 0, 2, 3, 4] = 1.0
+    tr.tv_add_grad(p, g, 1.0, 1.0, 1.0, dense_mode=False)
+    assert (g != 0).sum() == 1
+
+
+def test_adam_first_step_closed_form():
+    p = torch.randn(1000, dtype=torch.float64)
+    g = torch.randn(1000, dtype=torch.float64)
+    m, v = torch.zeros_like(p), torch.zeros_like(p)
+    p0 = p.clone()
+    tr.adam_step(p, g, m, v, 1, 0.1, 0.9, 0.99, 1e-8)
+    torch.testing.assert_close(p, p0 - 0.1 * g / (g.abs() + 1e-8))
+    # adam_upd places eps differently: -lr g / (|g| + eps / sqrt(1 - beta2))
+    p = p0.clone().float()
+    m, v = torch.zeros_like(p), torch.zeros_like(p)
+    tr.adam_upd(p, g.float(), m, v, 1, 0.9, 0.99, 0.1, 1e-8)
+    torch.testing.assert_close(p, (p0 - 0.1 * g / (g.abs() + 1e-8 / 0.1)).float(), rtol=1e-5, atol=1e-6)
+
+
+def test_segment_sum_brute_force():
+    idx = torch.tensor([0, 0, 1, 3, 3, 3, 5])
+    src = torch.randn(7, 3, dtype=torch.float64)
+    out = tr.segment_sum(src, idx, 7)
+    exp = torch.zeros(7, 3, dtype=torch.float64)
+    for i, k in enumerate(idx.tolist()):
+        exp[k] += src[i]
+    torch.testing.assert_close(out, exp)
+
+
+def test_view_pe_layout():
+    v = torch.tensor([[0.1, 0.2, 0.3]], dtype=torch.float64)
+    e = tr.view_pe(v, 2)
+    # [x, y, z, sin(x), sin(2x), sin(y), sin(2y), sin(z), sin(2z), cos(...)]
+    exp = [0.1, 0.2, 0.3] + [np.sin(f * c) for c in (0.1, 0.2, 0.3) for f in (1, 2)] + \
+          [np.cos(f * c) for c in (0.1, 0.2, 0.3) for f in (1, 2)]
+    torch.testing.assert_close(e[0], torch.tensor(exp, dtype=torch.float64))
+
+
+def test_planted_mistake_is_caught():
+    """A restatement with the x/z axes of the normal swapped (a plausible slip in the
+    zyx -> xyz reorder) must fail the linear-field check."""
+    g = linear_grid(9)
+    pts = torch.rand(50, 3, dtype=torch.float64) * 1.6 - 0.8
+    feat, grad = tr.sample_sdfs(pts, g, [1.0], MN, MX, 0.25)
+    wrong = grad  # zyx order left as is, i.e. no reorder
+    right = torch.cat([grad[:, [2]], grad[:, [1]], grad[:, [0]]], -1)
+    a = torch.tensor([0.3, -0.7, 1.1], dtype=torch.float64)
+    assert torch.allclose(right, a.expand_as(right))
+    assert not torch.allclose(wrong, a.expand_as(wrong))

@@ -1,0 +1,477 @@
+// Fused per-ray kernels of the fine training step (gfx950).
+//
+// The reference runs the fine forward as ~40 separate passes over materialised sample
+// arrays (lib/voxurf_fine.py:620-802): sample every step of every ray, compact the
+// in-box ones, gather the SDF at 7 points per sample, NeuS alpha, compact alpha > thres,
+// walk the transmittance per ray, compact weight > thres, then features and colour.
+// Here one wavefront owns one ray and does all of that in registers, 64 consecutive
+// steps per iteration, and stops once the transmittance walk has terminated (every
+// later sample has weight 0 and no gradient in the reference).  Only the surviving
+// samples (weight > thres) leave the kernel, in per-ray slots.  The backward recomputes
+// each ray the same way instead of storing every sample.
+//
+// The arithmetic of every sample is the drop-in ops' own (voxel.h), and the
+// transmittance walk is the reference's sequential float recurrence, so the survivor
+// set, weights and alphainv_last equal those of the op-by-op pipeline.
+#include "voxel.h"
+
+namespace vx {
+
+struct FusedGeom {
+  const float* sdf;       // [X, Y, Z] contiguous
+  GridDesc gs;
+  const float* k0;        // [C, X, Y, Z] with strides
+  GridDesc gk;
+  float mn[3], mx[3];
+  float stepdist, voxel_size, inv_s, thres, near_, far_;
+};
+
+__device__ __forceinline__ uint64_t lanemask_lt(int lane) {
+  return lane ? ((~0ull) >> (64 - lane)) : 0ull;
+}
+
+// One 64-step chunk of a ray: sample points, SDF + gradient, NeuS alpha, then the
+// exact sequential transmittance walk over the chunk's active samples.
+struct ChunkState {
+  float q[3];
+  float sdf, grad[3];
+  NeusFwd<float> nf;
+  float alpha, myT, w;
+  bool inb, act, inrange, surv;
+};
+
+__device__ __forceinline__ void run_chunk(const FusedGeom& G, const float* start, const float* dir,
+                                          const float* vd, int64_t n, int64_t base, int lane,
+                                          float& T, bool& stopped, ChunkState& cs) {
+  const int64_t s = base + lane;
+  const bool valid = s < n;
+  bool out = true;
+  if (valid) out = sample_point<float>(start, dir, G.mn, G.mx, G.stepdist, s, cs.q[0], cs.q[1], cs.q[2]);
+  cs.inb = valid && !out;
+  cs.alpha = 0.f;
+  cs.sdf = 0.f;
+  cs.grad[0] = cs.grad[1] = cs.grad[2] = 0.f;
+  if (cs.inb) {
+    sdf7<float>(G.sdf, G.gs, cs.q, G.mn, G.mx, G.voxel_size, cs.sdf, cs.grad, nullptr);
+    cs.nf = neus_fwd<float>(vd, cs.grad, cs.sdf, G.stepdist, G.inv_s);
+    cs.alpha = clip01(cs.nf.a);
+  }
+  cs.act = cs.inb && cs.alpha > G.thres;
+  uint64_t mm = __ballot(cs.act);
+  int stop_lane = 64;
+  cs.myT = 1.f;
+  while (mm) {
+    const int k = __ffsll((unsigned long long)mm) - 1;
+    mm &= mm - 1;
+    const float ak = __shfl(cs.alpha, k, 64);
+    if (lane == k) cs.myT = T;
+    T = (float)((double)T * (1.0 - (double)ak));
+    if ((double)T < 1e-3) {
+      stopped = true;
+      stop_lane = k;
+      break;
+    }
+  }
+  cs.inrange = cs.act && lane <= stop_lane;
+  cs.w = cs.myT * cs.alpha;
+  cs.surv = cs.inrange && cs.w > G.thres;
+}
+
+__device__ __forceinline__ void ray_setup(const FusedGeom& G, const float* o, const float* d,
+                                          float* start, float* dir, int64_t& n) {
+  float tmin, tmax;
+  ray_t_minmax<float>(o, d, G.mn, G.mx, G.near_, G.far_, tmin, tmax);
+  n = ray_n_samples<float>(d, tmin, tmax, G.stepdist);
+  ray_start_dir<float>(o, d, tmin, start, dir);
+}
+
+// Forward.  Outputs per ray: cnt (survivors), last (alphainv_last), stats
+// {m0 processed, m1 in the walk range, m0 of the whole ray (if count_all)};
+// per survivor k of ray r at slot r*cap + k: s4 = (x, y, z, w), g4 = (gx, gy, gz, step).
+__global__ void __launch_bounds__(256) k_fused_fwd(
+    FusedGeom G, const float* __restrict__ rays_o, const float* __restrict__ rays_d,
+    const float* __restrict__ viewdirs, int64_t n_rays, int cap, float4* __restrict__ s4,
+    float4* __restrict__ g4, int64_t* __restrict__ cnt_out, float* __restrict__ last_out,
+    int* __restrict__ stats, int count_all, int* __restrict__ overflow) {
+  const int lane = threadIdx.x & 63;
+  const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+  for (int64_t r = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6); r < n_rays;
+       r += waves) {
+    float start[3], dir[3];
+    int64_t n;
+    ray_setup(G, rays_o + 3 * r, rays_d + 3 * r, start, dir, n);
+    const float* vd = viewdirs + 3 * r;
+    float T = 1.f;
+    bool stopped = false;
+    int cnt = 0, m0 = 0, m1 = 0;
+    int64_t base = 0;
+    for (; base < n && !stopped; base += 64) {
+      ChunkState cs;
+      run_chunk(G, start, dir, vd, n, base, lane, T, stopped, cs);
+      m0 += __popcll(__ballot(cs.inb));
+      m1 += __popcll(__ballot(cs.inrange));
+      const uint64_t sm = __ballot(cs.surv);
+      if (cs.surv) {
+        const int k = cnt + __popcll(sm & lanemask_lt(lane));
+        if (k < cap) {
+          const int64_t slot = r * cap + k;
+          s4[slot] = make_float4(cs.q[0], cs.q[1], cs.q[2], cs.w);
+          g4[slot] = make_float4(cs.grad[0], cs.grad[1], cs.grad[2], (float)(base + lane));
+        }
+      }
+      cnt += __popcll(sm);
+    }
+    int m0_all = m0;
+    if (count_all) {
+      // in-box samples after termination (bbox tests only), for sample statistics
+      for (; base < n; base += 64) {
+        const int64_t s = base + lane;
+        float q[3];
+        bool out = true;
+        if (s < n) out = sample_point<float>(start, dir, G.mn, G.mx, G.stepdist, s, q[0], q[1], q[2]);
+        m0_all += __popcll(__ballot(s < n && !out));
+      }
+    }
+    if (lane == 0) {
+      if (cnt > cap) {
+        atomicOr(overflow, 1);
+        cnt = cap;
+      }
+      cnt_out[r] = cnt;
+      last_out[r] = T;
+      stats[3 * r + 0] = m0;
+      stats[3 * r + 1] = m1;
+      stats[3 * r + 2] = m0_all;
+    }
+  }
+}
+
+// Compaction + rgbnet input.  One wavefront per ray copies its survivors to
+// off[r] .. off[r] + cnt[r] and builds X = [k0 (C) | normal (3) | view PE (3 + 6 nf) | 0 pad].
+// normal = grad / (|grad| + 1e-5) (lib/voxurf_fine.py:766 with the 1e-5 of :810);
+// view PE = [v, sin(v_i f_j), cos(v_i f_j)] with f_j = 2^j (lib/voxurf_fine.py:694-695).
+__global__ void __launch_bounds__(256) k_fused_gather(
+    FusedGeom G, const float* __restrict__ viewdirs, int64_t n_rays, int cap, int n_freq,
+    const float4* __restrict__ s4, const float4* __restrict__ g4, const int64_t* __restrict__ cnt,
+    const int64_t* __restrict__ off_incl, int xstride, float* __restrict__ X,
+    float4* __restrict__ P4, float4* __restrict__ G4, int* __restrict__ rid) {
+  const int lane = threadIdx.x & 63;
+  const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+  const int C = G.gk.C;
+  const int npe = 3 + 6 * n_freq;
+  for (int64_t r = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6); r < n_rays;
+       r += waves) {
+    const int64_t c = cnt[r];
+    if (c == 0) continue;
+    const int64_t o = off_incl[r] - c;
+    // view PE of this ray, lane e holds element e
+    float pe = 0.f;
+    {
+      const float* v = viewdirs + 3 * r;
+      if (lane < 3) {
+        pe = v[lane];
+      } else if (lane < npe) {
+        const int e = lane - 3;
+        const int half = 3 * n_freq;
+        const int idx = e % half;  // (component, freq) in component-major order
+        const int comp = idx / n_freq, f = idx % n_freq;
+        const float arg = v[comp] * (float)(1 << f);
+        pe = (e < half) ? sinf(arg) : cosf(arg);
+      }
+    }
+    for (int64_t k = 0; k < c; ++k) {
+      // one survivor per iteration; lanes cooperate on the feature row
+      const float4 a = s4[r * cap + k];
+      const float4 b = g4[r * cap + k];
+      const int64_t j = o + k;
+      const float q[3] = {a.x, a.y, a.z};
+      Tri<float> t;
+      t.init(world_to_index(q[0], G.mn[0], G.mx[0] - G.mn[0], G.gk.X),
+             world_to_index(q[1], G.mn[1], G.mx[1] - G.mn[1], G.gk.Y),
+             world_to_index(q[2], G.mn[2], G.mx[2] - G.mn[2], G.gk.Z));
+      float val = 0.f;
+      if (lane < C) {
+        val = tri_gather(G.k0, G.gk, t, lane);
+      } else if (lane < C + 3) {
+        const float gx = b.x, gy = b.y, gz = b.z;
+        const float nrm = sqrtf(gx * gx + gy * gy + gz * gz) + 1e-5f;
+        const float comp = (lane == C) ? gx : (lane == C + 1 ? gy : gz);
+        val = comp / nrm;
+      }
+      const float pv = __shfl(pe, (lane - C - 3) & 63, 64);
+      if (lane >= C + 3 && lane < C + 3 + npe) val = pv;
+      if (lane < xstride) X[j * xstride + lane] = val;
+      if (lane == 0) {
+        P4[j] = a;
+        G4[j] = b;
+        rid[j] = (int)r;
+      }
+    }
+  }
+}
+
+// Compositing, loss and the colour-side adjoints, one thread per ray:
+//   rgb_marched = sum_j w_j sigmoid(logit_j) + last * bg          (lib/voxurf_fine.py:761-763)
+//   loss_r = sum_c (rgb_marched - target)^2,  g = 2 (rgb_marched - target) / (3 N)
+//   dlogit_j = (w_j g) * rgb (1 - rgb);  gw_j = <g, rgb_j>;  glast = <g, bg>;
+//   S_r = sum_j gw_j w_j  (total of the transmittance adjoint's suffix sums)
+__global__ void k_fused_composite(int64_t n_rays, const int64_t* __restrict__ cnt,
+                                  const int64_t* __restrict__ off_incl,
+                                  const float* __restrict__ logits, int lstride,
+                                  const float4* __restrict__ P4, const float* __restrict__ last,
+                                  const float* __restrict__ target, float bg, float inv_count,
+                                  float* __restrict__ rgb_out, float* __restrict__ dlogit,
+                                  float* __restrict__ gw, float* __restrict__ glast,
+                                  float* __restrict__ S, float* __restrict__ loss_r) {
+  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n_rays;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t c = cnt[r], o = off_incl[r] - c;
+    float acc[3] = {0.f, 0.f, 0.f};
+    for (int64_t j = o; j < o + c; ++j) {
+      const float w = P4[j].w;
+      for (int ch = 0; ch < 3; ++ch) {
+        const float rgb = 1.f / (1.f + expf(-logits[j * lstride + ch]));
+        acc[ch] += w * rgb;
+      }
+    }
+    float g[3], l = 0.f;
+    for (int ch = 0; ch < 3; ++ch) {
+      const float v = acc[ch] + last[r] * bg;
+      rgb_out[3 * r + ch] = v;
+      const float d = v - target[3 * r + ch];
+      l += d * d;
+      g[ch] = 2.f * d * inv_count;
+    }
+    loss_r[r] = l;
+    glast[r] = (g[0] + g[1] + g[2]) * bg;
+    float s = 0.f;
+    for (int64_t j = o; j < o + c; ++j) {
+      const float w = P4[j].w;
+      float gwj = 0.f;
+      for (int ch = 0; ch < 3; ++ch) {
+        const float rgb = 1.f / (1.f + expf(-logits[j * lstride + ch]));
+        gwj += g[ch] * rgb;
+        dlogit[j * 3 + ch] = (w * g[ch]) * (rgb * (1.f - rgb));
+      }
+      gw[j] = gwj;
+      s += gwj * w;
+    }
+    S[r] = s;
+  }
+}
+
+// Feature-side adjoint per survivor: dX[:, 0:C] -> k0 grad (trilinear scatter);
+// dX[:, C:C+3] (normal) -> d grad through n = g / (|g| + 1e-5); result stored in dgf.
+__global__ void k_fused_feat_bwd(FusedGeom G, int64_t m, const float* __restrict__ dX, int dxs,
+                                 const float4* __restrict__ P4, const float4* __restrict__ G4,
+                                 float* __restrict__ k0_grad, float* __restrict__ dgf) {
+  const int C = G.gk.C;
+  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < m;
+       j += (int64_t)gridDim.x * blockDim.x) {
+    const float4 a = P4[j];
+    Tri<float> t;
+    t.init(world_to_index(a.x, G.mn[0], G.mx[0] - G.mn[0], G.gk.X),
+           world_to_index(a.y, G.mn[1], G.mx[1] - G.mn[1], G.gk.Y),
+           world_to_index(a.z, G.mn[2], G.mx[2] - G.mn[2], G.gk.Z));
+    for (int ch = 0; ch < C; ++ch) tri_scatter(k0_grad, G.gk, t, ch, dX[j * dxs + ch]);
+    const float4 b = G4[j];
+    const float gvec[3] = {b.x, b.y, b.z};
+    const float dn[3] = {dX[j * dxs + C], dX[j * dxs + C + 1], dX[j * dxs + C + 2]};
+    const float nr = sqrtf(gvec[0] * gvec[0] + gvec[1] * gvec[1] + gvec[2] * gvec[2]);
+    const float den = nr + 1e-5f;
+    const float dot = gvec[0] * dn[0] + gvec[1] * dn[1] + gvec[2] * dn[2];
+    const float k = nr > 0.f ? dot / (nr * den) : 0.f;
+    for (int i = 0; i < 3; ++i) dgf[3 * j + i] = (dn[i] - gvec[i] * k) / den;
+  }
+}
+
+// Backward, one wavefront per ray, recomputing the forward walk:
+//   dalpha_i = gw_i T_i - back_i / (1 - alpha_i + 1e-10)   on the walk range,
+//   back_i = glast * last + S_r - sum_{survivors j <= i} gw_j w_j
+// (render_utils_kernel.cu:654-677 with the suffix sum written as total - prefix), then
+// the NeuS adjoint and the 7-point stencil scatter into the SDF gradient.
+__global__ void __launch_bounds__(256) k_fused_bwd(
+    FusedGeom G, const float* __restrict__ rays_o, const float* __restrict__ rays_d,
+    const float* __restrict__ viewdirs, int64_t n_rays, const int64_t* __restrict__ cnt,
+    const int64_t* __restrict__ off_incl, const float* __restrict__ last,
+    const float* __restrict__ gw, const float* __restrict__ glast, const float* __restrict__ S,
+    const float* __restrict__ dgf, float* __restrict__ sdf_grad) {
+  const int lane = threadIdx.x & 63;
+  const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+  for (int64_t r = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6); r < n_rays;
+       r += waves) {
+    float start[3], dir[3];
+    int64_t n;
+    ray_setup(G, rays_o + 3 * r, rays_d + 3 * r, start, dir, n);
+    const float* vd = viewdirs + 3 * r;
+    const int64_t c_r = cnt[r];
+    const int64_t o = off_incl[r] - c_r;
+    const float base_back = glast[r] * last[r] + S[r];
+    float T = 1.f, prefix = 0.f;
+    bool stopped = false;
+    int k_surv = 0;
+    for (int64_t base = 0; base < n && !stopped; base += 64) {
+      ChunkState cs;
+      run_chunk(G, start, dir, vd, n, base, lane, T, stopped, cs);
+      const uint64_t sm = __ballot(cs.surv);
+      float gwl = 0.f, dg[3] = {0.f, 0.f, 0.f};
+      const int64_t kk = k_surv + __popcll(sm & lanemask_lt(lane));
+      if (cs.surv && kk < c_r) {
+        const int64_t j = o + kk;
+        gwl = gw[j];
+        dg[0] = dgf[3 * j];
+        dg[1] = dgf[3 * j + 1];
+        dg[2] = dgf[3 * j + 2];
+      }
+      k_surv += __popcll(sm);
+      // inclusive prefix of gw*w over survivors within the chunk
+      const float c = gwl * cs.w;
+      float incl = c;
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const float u = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += u;
+      }
+      const float back = base_back - (prefix + incl);
+      prefix += __shfl(incl, 63, 64);
+      if (cs.inrange) {
+        const double den = (double)(1.f - cs.alpha) + 1e-10;
+        const float da = (float)((double)(gwl * cs.myT) - (double)back / den);
+        float gsdf, gg[3];
+        neus_bwd<float>(vd, cs.nf, G.stepdist, G.inv_s, da, gsdf, gg);
+        gg[0] += dg[0];
+        gg[1] += dg[1];
+        gg[2] += dg[2];
+        sdf7_scatter<float>(sdf_grad, G.gs, cs.q, G.mn, G.mx, G.voxel_size, gsdf, gg, nullptr);
+      }
+    }
+  }
+}
+
+__global__ void k_sum_f32(const float* __restrict__ x, int64_t n, float* __restrict__ out) {
+  float s = 0.f;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    s += x[i];
+  s = wave_sum_f32(s);
+  if ((threadIdx.x & 63) == 0) unsafeAtomicAdd(out, s);
+}
+
+}  // namespace vx
+
+using namespace vx;
+
+static int fused_geom(FusedGeom& G, const float* sdf, const int64_t* sdf_geom, const float* k0,
+                      const int64_t* k0_geom, const float* box, const float* params) {
+  // box = {mn x,y,z, mx x,y,z}; params = {stepdist, voxel_size, inv_s, thres, near, far}
+  G.sdf = sdf;
+  G.k0 = k0;
+  auto desc = [](const int64_t* s, GridDesc& g) {
+    g.C = (int)s[0];
+    g.X = (int)s[1];
+    g.Y = (int)s[2];
+    g.Z = (int)s[3];
+    g.sc = s[4];
+    g.sx = s[5];
+    g.sy = s[6];
+    g.sz = s[7];
+  };
+  desc(sdf_geom, G.gs);
+  if (k0_geom) desc(k0_geom, G.gk);
+  else G.gk = G.gs;
+  VX_CHECK_ARG(G.gs.C == 1, "fused: the SDF grid must have one channel");
+  for (int i = 0; i < 3; ++i) {
+    G.mn[i] = box[i];
+    G.mx[i] = box[3 + i];
+  }
+  G.stepdist = params[0];
+  G.voxel_size = params[1];
+  G.inv_s = params[2];
+  G.thres = params[3];
+  G.near_ = params[4];
+  G.far_ = params[5];
+  return VX_OK;
+}
+
+extern "C" {
+
+int vx_fused_fine_fwd(const float* sdf, const int64_t* sdf_geom, const float* box,
+                      const float* params, const float* rays_o, const float* rays_d,
+                      const float* viewdirs, int64_t n_rays, int cap, void* s4, void* g4,
+                      int64_t* cnt, float* last, int* stats, int count_all, int* overflow,
+                      void* stream) {
+  FusedGeom G;
+  if (int st = fused_geom(G, sdf, sdf_geom, nullptr, nullptr, box, params)) return st;
+  VX_CHECK_ARG(n_rays >= 0 && cap >= 1, "fused_fine_fwd: bad sizes");
+  if (n_rays == 0) return VX_OK;
+  k_fused_fwd<<<grid_for(n_rays, 4, 1 << 16), 256, 0, (hipStream_t)stream>>>(
+      G, rays_o, rays_d, viewdirs, n_rays, cap, (float4*)s4, (float4*)g4, cnt, last, stats,
+      count_all, overflow);
+  return launch_status("fused_fine_fwd");
+}
+
+int vx_fused_fine_gather(const float* k0, const int64_t* k0_geom, const int64_t* sdf_geom,
+                         const float* box, const float* params, const float* viewdirs,
+                         int64_t n_rays, int cap, int n_freq, const void* s4, const void* g4,
+                         const int64_t* cnt, const int64_t* off_incl, int xstride, float* X,
+                         void* P4, void* G4, int* rid, void* stream) {
+  FusedGeom G;
+  if (int st = fused_geom(G, nullptr, sdf_geom, k0, k0_geom, box, params)) return st;
+  VX_CHECK_ARG(G.gk.C + 3 + 3 + 6 * n_freq <= xstride && xstride <= 64,
+               "fused_fine_gather: feature row does not fit xstride");
+  if (n_rays == 0) return VX_OK;
+  k_fused_gather<<<grid_for(n_rays, 4, 1 << 16), 256, 0, (hipStream_t)stream>>>(
+      G, viewdirs, n_rays, cap, n_freq, (const float4*)s4, (const float4*)g4, cnt, off_incl,
+      xstride, X, (float4*)P4, (float4*)G4, rid);
+  return launch_status("fused_fine_gather");
+}
+
+int vx_fused_fine_composite(int64_t n_rays, const int64_t* cnt, const int64_t* off_incl,
+                            const float* logits, int lstride, const void* P4, const float* last,
+                            const float* target, float bg, float* rgb, float* dlogit, float* gw,
+                            float* glast, float* S, float* loss_r, void* stream) {
+  VX_CHECK_ARG(n_rays >= 0, "fused_fine_composite: n_rays < 0");
+  if (n_rays == 0) return VX_OK;
+  const float inv_count = 1.f / (3.f * (float)n_rays);
+  k_fused_composite<<<grid_for(n_rays, 64), 64, 0, (hipStream_t)stream>>>(
+      n_rays, cnt, off_incl, logits, lstride, (const float4*)P4, last, target, bg, inv_count, rgb,
+      dlogit, gw, glast, S, loss_r);
+  return launch_status("fused_fine_composite");
+}
+
+int vx_fused_fine_feat_bwd(const int64_t* k0_geom, const int64_t* sdf_geom, const float* box,
+                           const float* params, int64_t m, const float* dX, int dxs,
+                           const void* P4, const void* G4, float* k0_grad, float* dgf,
+                           void* stream) {
+  FusedGeom G;
+  if (int st = fused_geom(G, nullptr, sdf_geom, nullptr, k0_geom, box, params)) return st;
+  if (m == 0) return VX_OK;
+  k_fused_feat_bwd<<<grid_for(m, 256), 256, 0, (hipStream_t)stream>>>(
+      G, m, dX, dxs, (const float4*)P4, (const float4*)G4, k0_grad, dgf);
+  return launch_status("fused_fine_feat_bwd");
+}
+
+int vx_fused_fine_bwd(const float* sdf, const int64_t* sdf_geom, const float* box,
+                      const float* params, const float* rays_o, const float* rays_d,
+                      const float* viewdirs, int64_t n_rays, const int64_t* cnt,
+                      const int64_t* off_incl, const float* last, const float* gw,
+                      const float* glast, const float* S, const float* dgf, float* sdf_grad,
+                      void* stream) {
+  FusedGeom G;
+  if (int st = fused_geom(G, sdf, sdf_geom, nullptr, nullptr, box, params)) return st;
+  if (n_rays == 0) return VX_OK;
+  k_fused_bwd<<<grid_for(n_rays, 4, 1 << 16), 256, 0, (hipStream_t)stream>>>(
+      G, rays_o, rays_d, viewdirs, n_rays, cnt, off_incl, last, gw, glast, S, dgf, sdf_grad);
+  return launch_status("fused_fine_bwd");
+}
+
+int vx_sum_f32(const float* x, int64_t n, float* out, void* stream) {
+  VX_CHECK_ARG(n >= 0, "sum: n < 0");
+  hipStream_t st = (hipStream_t)stream;
+  vx_zero(VX_F32, out, 1, stream);
+  if (n == 0) return launch_status("sum");
+  k_sum_f32<<<grid_for(n, 256, 1024), 256, 0, st>>>(x, n, out);
+  return launch_status("sum");
+}
+
+}  // extern "C"

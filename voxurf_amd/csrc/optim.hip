@@ -1,0 +1,166 @@
+// Optimiser kernels (gfx950): the reference's fused adam_upd family and the per-step
+// Adam the training loop actually runs (lib/utils.py:154-199), each one streaming pass
+// over (param, grad, exp_avg, exp_avg_sq) with 16-byte vector accesses.  The training
+// Adam also clears the gradient it has consumed, which replaces the per-step dense
+// zero-fill of the gradient buffers.
+#include <math.h>
+
+#include "common.h"
+
+namespace vx {
+
+// adam_upd_kernel.cu:8-58.  mode 0: plain, 1: masked (skip grad == 0), perlr != null:
+// per-element learning-rate factor.
+template <typename T>
+__global__ void k_adam_upd(T* __restrict__ p, const T* __restrict__ g, T* __restrict__ m,
+                           T* __restrict__ v, const T* __restrict__ perlr, int64_t n,
+                           float step_size, float b1, float b2, float eps, int masked) {
+#pragma clang fp contract(off)
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const T gi = g[i];
+    if (masked && gi == (T)0) continue;
+    const T mi = (T)b1 * m[i] + (T)(1 - b1) * gi;
+    const T vi = (T)b2 * v[i] + (T)(1 - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const T ss = perlr ? (T)step_size * perlr[i] : (T)step_size;
+    p[i] -= ss * mi / (sqrt(vi) + (T)eps);
+  }
+}
+
+struct AdamCoef {
+  float b1, omb1, b2, omb2, sqrt_bc2, eps, neg_step;
+};
+
+// One element of lib/utils.py:184-199:
+//   m = m*b1 + (1-b1)*g;  v = v*b2 + (1-b2)*g*g
+//   denom = sqrt(v) / sqrt(1-b2^t) + eps;  p += -(lr/(1-b1^t)) * (m [* per_lr]) / denom
+__device__ __forceinline__ void adam_elem(float& p, float g, float& m, float& v, float plr,
+                                          const AdamCoef& c) {
+  m = m * c.b1 + c.omb1 * g;
+  v = v * c.b2 + (c.omb2 * g) * g;
+  const float denom = sqrtf(v) / c.sqrt_bc2 + c.eps;
+  p = p + c.neg_step * ((m * plr) / denom);
+}
+
+__global__ void __launch_bounds__(256) k_adam_step_f32(float* __restrict__ p, float* __restrict__ g,
+                                                       float* __restrict__ m, float* __restrict__ v,
+                                                       const float* __restrict__ perlr, int64_t n,
+                                                       AdamCoef c, int clear, int vec) {
+  const int64_t n4 = vec ? (n >> 2) : 0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  float4* p4 = reinterpret_cast<float4*>(p);
+  float4* g4 = reinterpret_cast<float4*>(g);
+  float4* m4 = reinterpret_cast<float4*>(m);
+  float4* v4 = reinterpret_cast<float4*>(v);
+  const float4* l4 = reinterpret_cast<const float4*>(perlr);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n4; i += stride) {
+    float4 pp = p4[i], gg = g4[i], mm = m4[i], vv = v4[i];
+    const float4 ll = perlr ? l4[i] : make_float4(1.f, 1.f, 1.f, 1.f);
+    adam_elem(pp.x, gg.x, mm.x, vv.x, ll.x, c);
+    adam_elem(pp.y, gg.y, mm.y, vv.y, ll.y, c);
+    adam_elem(pp.z, gg.z, mm.z, vv.z, ll.z, c);
+    adam_elem(pp.w, gg.w, mm.w, vv.w, ll.w, c);
+    p4[i] = pp;
+    m4[i] = mm;
+    v4[i] = vv;
+    if (clear) g4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  for (int64_t i = (n4 << 2) + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride) {
+    float pp = p[i], mm = m[i], vv = v[i];
+    adam_elem(pp, g[i], mm, vv, perlr ? perlr[i] : 1.f, c);
+    p[i] = pp;
+    m[i] = mm;
+    v[i] = vv;
+    if (clear) g[i] = 0.f;
+  }
+}
+
+__global__ void k_adam_step_f64(double* __restrict__ p, double* __restrict__ g,
+                                double* __restrict__ m, double* __restrict__ v,
+                                const double* __restrict__ perlr, int64_t n, double b1, double b2,
+                                double sqrt_bc2, double eps, double neg_step, int clear) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double gi = g[i];
+    const double mi = m[i] * b1 + (1 - b1) * gi;
+    const double vi = v[i] * b2 + ((1 - b2) * gi) * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const double denom = sqrt(vi) / sqrt_bc2 + eps;
+    p[i] = p[i] + neg_step * ((mi * (perlr ? perlr[i] : 1.0)) / denom);
+    if (clear) g[i] = 0;
+  }
+}
+
+}  // namespace vx
+
+using namespace vx;
+
+extern "C" {
+
+int vx_adam_upd(int dtype, void* param, const void* grad, void* exp_avg, void* exp_avg_sq,
+                const void* perlr, int64_t n, int step, float beta1, float beta2, float lr,
+                float eps, int masked, void* stream) {
+  VX_CHECK_ARG(n >= 0, "adam_upd: n < 0");
+  VX_CHECK_ARG(step >= 1, "adam_upd: step must be >= 1");
+  if (n == 0) return VX_OK;
+  // adam_upd_kernel.cu:72, in float as the reference's host code
+  const float step_size =
+      lr * sqrtf(1 - powf(beta2, (float)step)) / (1 - powf(beta1, (float)step));
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == VX_F32)
+    k_adam_upd<float><<<grid_for(n, 256), 256, 0, st>>>(
+        (float*)param, (const float*)grad, (float*)exp_avg, (float*)exp_avg_sq,
+        (const float*)perlr, n, step_size, beta1, beta2, eps, masked);
+  else if (dtype == VX_F64)
+    k_adam_upd<double><<<grid_for(n, 256), 256, 0, st>>>(
+        (double*)param, (const double*)grad, (double*)exp_avg, (double*)exp_avg_sq,
+        (const double*)perlr, n, step_size, beta1, beta2, eps, masked);
+  else {
+    set_error("unsupported dtype code %d", dtype);
+    return VX_ERR_DTYPE;
+  }
+  return launch_status("adam_upd");
+}
+
+int vx_adam_step(int dtype, void* param, void* grad, void* exp_avg, void* exp_avg_sq,
+                 const void* per_lr, int64_t n, int step, double lr, double beta1, double beta2,
+                 double eps, int clear_grad, void* stream) {
+  VX_CHECK_ARG(n >= 0, "adam_step: n < 0");
+  VX_CHECK_ARG(step >= 1, "adam_step: step must be >= 1");
+  if (n == 0) return VX_OK;
+  const double bc1 = 1.0 - pow(beta1, (double)step);
+  const double bc2 = 1.0 - pow(beta2, (double)step);
+  const double step_size = lr / bc1;
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == VX_F32) {
+    // 16-byte vector path when every buffer is 16-byte aligned, scalar otherwise
+    const int vec = (((uintptr_t)param | (uintptr_t)grad | (uintptr_t)exp_avg |
+                      (uintptr_t)exp_avg_sq | (uintptr_t)per_lr) % 16) == 0;
+    AdamCoef c;
+    // Python scalars become float32 when applied to float32 tensors.
+    c.b1 = (float)beta1;
+    c.omb1 = (float)(1.0 - beta1);
+    c.b2 = (float)beta2;
+    c.omb2 = (float)(1.0 - beta2);
+    c.sqrt_bc2 = (float)sqrt(bc2);
+    c.eps = (float)eps;
+    c.neg_step = (float)(-step_size);
+    const int64_t work = vec ? n / 4 + (n & 3) : n;
+    k_adam_step_f32<<<grid_for(work, 256, 8192), 256, 0, st>>>(
+        (float*)param, (float*)grad, (float*)exp_avg, (float*)exp_avg_sq, (const float*)per_lr,
+        n, c, clear_grad, vec);
+  } else if (dtype == VX_F64) {
+    k_adam_step_f64<<<grid_for(n, 256), 256, 0, st>>>(
+        (double*)param, (double*)grad, (double*)exp_avg, (double*)exp_avg_sq,
+        (const double*)per_lr, n, beta1, beta2, sqrt(bc2), eps, -step_size, clear_grad);
+  } else {
+    set_error("unsupported dtype code %d", dtype);
+    return VX_ERR_DTYPE;
+  }
+  return launch_status("adam_step");
+}
+
+}  // extern "C"

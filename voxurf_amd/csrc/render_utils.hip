@@ -9,7 +9,7 @@
 #include <stdarg.h>
 #include <string.h>
 
-#include "common.h"
+#include "voxel.h"
 
 namespace vx {
 
@@ -25,41 +25,6 @@ void set_error(const char* fmt, ...) {
 // ---------------------------------------------------------------------------------
 // Per-ray geometry (render_utils_kernel.cu:12-79).  T is the tensor scalar type; the
 // reference keeps several temporaries in `float` regardless of T, reproduced here.
-
-template <typename T>
-__device__ __forceinline__ void ray_t_minmax(const T* o, const T* d, const T* xyz_min,
-                                             const T* xyz_max, float near_, float far_,
-                                             float& tmin, float& tmax) {
-#pragma clang fp contract(off)
-  // `(d==0) ? 1e-6 : d` has type double in the reference; the result is stored in a float.
-  const float vx = (d[0] == 0) ? (float)1e-6 : (float)d[0];
-  const float vy = (d[1] == 0) ? (float)1e-6 : (float)d[1];
-  const float vz = (d[2] == 0) ? (float)1e-6 : (float)d[2];
-  const float ax = (float)((xyz_max[0] - o[0]) / (T)vx);
-  const float ay = (float)((xyz_max[1] - o[1]) / (T)vy);
-  const float az = (float)((xyz_max[2] - o[2]) / (T)vz);
-  const float bx = (float)((xyz_min[0] - o[0]) / (T)vx);
-  const float by = (float)((xyz_min[1] - o[1]) / (T)vy);
-  const float bz = (float)((xyz_min[2] - o[2]) / (T)vz);
-  tmin = fmaxf(fminf(fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fminf(az, bz)), far_), near_);
-  tmax = fmaxf(fminf(fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz)), far_), near_);
-}
-
-template <typename T>
-__device__ __forceinline__ float ray_norm(const T* d) {
-#pragma clang fp contract(off)
-  const T s = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-  return (float)sqrt(s);
-}
-
-template <typename T>
-__device__ __forceinline__ int64_t ray_n_samples(const T* d, T tmin, T tmax, float stepdist) {
-#pragma clang fp contract(off)
-  const float rnorm = ray_norm(d);
-  const T q = (tmax - tmin) * (T)rnorm / (T)stepdist;
-  const double c = ceil((double)q);
-  return (int64_t)(c > 1.0 ? c : 1.0);
-}
 
 template <typename T>
 __global__ void k_infer_t_minmax(const T* __restrict__ rays_o, const T* __restrict__ rays_d,
@@ -82,18 +47,6 @@ __global__ void k_infer_n_samples(const T* __restrict__ rays_d, const T* __restr
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n;
        r += (int64_t)gridDim.x * blockDim.x)
     n_samples[r] = ray_n_samples<T>(rays_d + 3 * r, t_min[r], t_max[r], stepdist);
-}
-
-template <typename T>
-__device__ __forceinline__ void ray_start_dir(const T* o, const T* d, T tmin, T* start, T* dir) {
-#pragma clang fp contract(off)
-  const float rnorm = ray_norm(d);
-  start[0] = o[0] + d[0] * tmin;
-  start[1] = o[1] + d[1] * tmin;
-  start[2] = o[2] + d[2] * tmin;
-  dir[0] = d[0] / (T)rnorm;
-  dir[1] = d[1] / (T)rnorm;
-  dir[2] = d[2] / (T)rnorm;
 }
 
 template <typename T>
@@ -126,21 +79,6 @@ __global__ void k_sample_count(const T* __restrict__ rays_o, const T* __restrict
     t_max[r] = (T)b;
     n_steps[r] = ray_n_samples<T>(rays_d + 3 * r, (T)a, (T)b, stepdist);
   }
-}
-
-// Point of step `s` on a ray (render_utils_kernel.cu:184-192): the coordinates are
-// rounded to float whatever T is, and the bbox test compares in T.
-template <typename T>
-__device__ __forceinline__ bool sample_point(const T* start, const T* dir, const T* xyz_min,
-                                             const T* xyz_max, float stepdist, int64_t s,
-                                             float& px, float& py, float& pz) {
-#pragma clang fp contract(off)
-  const float dist = stepdist * (float)(int)s;
-  px = (float)(start[0] + dir[0] * (T)dist);
-  py = (float)(start[1] + dir[1] * (T)dist);
-  pz = (float)(start[2] + dir[2] * (T)dist);
-  return (xyz_min[0] > (T)px) | (xyz_min[1] > (T)py) | (xyz_min[2] > (T)pz) |
-         (xyz_max[0] < (T)px) | (xyz_max[1] < (T)py) | (xyz_max[2] < (T)pz);
 }
 
 // Phase 2: one wavefront per ray writes the ray's contiguous segment, so every store

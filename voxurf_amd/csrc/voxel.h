@@ -1,0 +1,350 @@
+// Device helpers shared by the drop-in ops and the fused per-ray kernels, so both
+// paths evaluate every sample with the same operations in the same order.
+#pragma once
+#include "common.h"
+
+namespace vx {
+
+// ---- ray geometry (lib/cuda/render_utils_kernel.cu:12-79, 184-192) ----------------
+
+template <typename T>
+__device__ __forceinline__ void ray_t_minmax(const T* o, const T* d, const T* xyz_min,
+                                             const T* xyz_max, float near_, float far_,
+                                             float& tmin, float& tmax) {
+#pragma clang fp contract(off)
+  // `(d==0) ? 1e-6 : d` has type double in the reference; the result is stored in a float.
+  const float vx = (d[0] == 0) ? (float)1e-6 : (float)d[0];
+  const float vy = (d[1] == 0) ? (float)1e-6 : (float)d[1];
+  const float vz = (d[2] == 0) ? (float)1e-6 : (float)d[2];
+  const float ax = (float)((xyz_max[0] - o[0]) / (T)vx);
+  const float ay = (float)((xyz_max[1] - o[1]) / (T)vy);
+  const float az = (float)((xyz_max[2] - o[2]) / (T)vz);
+  const float bx = (float)((xyz_min[0] - o[0]) / (T)vx);
+  const float by = (float)((xyz_min[1] - o[1]) / (T)vy);
+  const float bz = (float)((xyz_min[2] - o[2]) / (T)vz);
+  tmin = fmaxf(fminf(fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fminf(az, bz)), far_), near_);
+  tmax = fmaxf(fminf(fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz)), far_), near_);
+}
+
+template <typename T>
+__device__ __forceinline__ float ray_norm(const T* d) {
+#pragma clang fp contract(off)
+  const T s = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  return (float)sqrt(s);
+}
+
+template <typename T>
+__device__ __forceinline__ int64_t ray_n_samples(const T* d, T tmin, T tmax, float stepdist) {
+#pragma clang fp contract(off)
+  const float rnorm = ray_norm(d);
+  const T q = (tmax - tmin) * (T)rnorm / (T)stepdist;
+  const double c = ceil((double)q);
+  return (int64_t)(c > 1.0 ? c : 1.0);
+}
+
+template <typename T>
+__device__ __forceinline__ void ray_start_dir(const T* o, const T* d, T tmin, T* start, T* dir) {
+#pragma clang fp contract(off)
+  const float rnorm = ray_norm(d);
+  start[0] = o[0] + d[0] * tmin;
+  start[1] = o[1] + d[1] * tmin;
+  start[2] = o[2] + d[2] * tmin;
+  dir[0] = d[0] / (T)rnorm;
+  dir[1] = d[1] / (T)rnorm;
+  dir[2] = d[2] / (T)rnorm;
+}
+
+// Point of step `s` on a ray (render_utils_kernel.cu:184-192): the coordinates are
+// rounded to float whatever T is, and the bbox test compares in T.
+template <typename T>
+__device__ __forceinline__ bool sample_point(const T* start, const T* dir, const T* xyz_min,
+                                             const T* xyz_max, float stepdist, int64_t s,
+                                             float& px, float& py, float& pz) {
+#pragma clang fp contract(off)
+  const float dist = stepdist * (float)(int)s;
+  px = (float)(start[0] + dir[0] * (T)dist);
+  py = (float)(start[1] + dir[1] * (T)dist);
+  pz = (float)(start[2] + dir[2] * (T)dist);
+  return (xyz_min[0] > (T)px) | (xyz_min[1] > (T)py) | (xyz_min[2] > (T)pz) |
+         (xyz_max[0] < (T)px) | (xyz_max[1] < (T)py) | (xyz_max[2] < (T)pz);
+}
+
+
+// ---- trilinear lookups on dense grids -----------------------------------------------
+
+struct GridDesc {
+  int64_t sc, sx, sy, sz;  // element strides
+  int C, X, Y, Z;
+};
+
+// World coordinate -> continuous index along an axis of `size` lattice points spanning
+// [mn, mx]:  n = ((x - mn) / (mx - mn)) * 2 - 1;  i = ((n + 1) / 2) * (size - 1)
+// (lib/grid.py:53 followed by grid_sample's align_corners unnormalisation).
+template <typename T>
+__device__ __forceinline__ T world_to_index(T x, T mn, T ext, int size) {
+#pragma clang fp contract(off)
+  const T n = ((x - mn) / ext) * (T)2 - (T)1;
+  return ((n + (T)1) / (T)2) * (T)(size - 1);
+}
+
+// The 8 corners of a trilinear lookup at continuous index (fx, fy, fz).
+template <typename T>
+struct Tri {
+  int x0, y0, z0;
+  T wx[2], wy[2], wz[2];
+  __device__ __forceinline__ void init(T fx, T fy, T fz) {
+#pragma clang fp contract(off)
+    const T ffx = floor(fx), ffy = floor(fy), ffz = floor(fz);
+    x0 = sat_f2i(ffx);
+    y0 = sat_f2i(ffy);
+    z0 = sat_f2i(ffz);
+    wx[0] = (ffx + (T)1) - fx;
+    wx[1] = fx - ffx;
+    wy[0] = (ffy + (T)1) - fy;
+    wy[1] = fy - ffy;
+    wz[0] = (ffz + (T)1) - fz;
+    wz[1] = fz - ffz;
+  }
+  // weight of corner (bx, by, bz); PyTorch multiplies W-, H-, then D-axis terms, i.e.
+  // z, y, x in world axes.
+  __device__ __forceinline__ T w(int bx, int by, int bz) const {
+#pragma clang fp contract(off)
+    return wz[bz] * wy[by] * wx[bx];
+  }
+};
+
+template <typename T>
+__device__ __forceinline__ bool in_grid(int x, int y, int z, const GridDesc& g) {
+  return (unsigned)x < (unsigned)g.X && (unsigned)y < (unsigned)g.Y && (unsigned)z < (unsigned)g.Z;
+}
+
+// Gather one channel: sum over corners in grid_sample's order (x outer, z inner).
+template <typename T>
+__device__ __forceinline__ T tri_gather(const T* __restrict__ grid, const GridDesc& g,
+                                        const Tri<T>& t, int c) {
+#pragma clang fp contract(off)
+  T acc = 0;
+  for (int bx = 0; bx < 2; ++bx)
+    for (int by = 0; by < 2; ++by)
+      for (int bz = 0; bz < 2; ++bz) {
+        const int x = t.x0 + bx, y = t.y0 + by, z = t.z0 + bz;
+        if (in_grid<T>(x, y, z, g))
+          acc += grid[c * g.sc + x * g.sx + y * g.sy + z * g.sz] * t.w(bx, by, bz);
+      }
+  return acc;
+}
+
+template <typename T>
+__device__ __forceinline__ void atomic_add(T* p, T v) {
+  unsafeAtomicAdd(p, v);
+}
+
+template <typename T>
+__device__ __forceinline__ void tri_scatter(T* __restrict__ grad, const GridDesc& g,
+                                            const Tri<T>& t, int c, T go) {
+#pragma clang fp contract(off)
+  if (go == (T)0) return;
+  for (int bx = 0; bx < 2; ++bx)
+    for (int by = 0; by < 2; ++by)
+      for (int bz = 0; bz < 2; ++bz) {
+        const int x = t.x0 + bx, y = t.y0 + by, z = t.z0 + bz;
+        if (in_grid<T>(x, y, z, g))
+          atomic_add(&grad[c * g.sc + x * g.sx + y * g.sy + z * g.sz], t.w(bx, by, bz) * go);
+      }
+}
+
+template <typename T>
+__device__ __forceinline__ T roundtrip_index(T a, int size) {
+#pragma clang fp contract(off)
+  const T n = (a / (T)(size - 1)) * (T)2 - (T)1;
+  return ((n + (T)1) / (T)2) * (T)(size - 1);
+}
+
+template <typename T>
+__device__ __forceinline__ T clampT(T v, T lo, T hi) {
+  return v < lo ? lo : (v > hi ? hi : v);
+}
+
+// Continuous (zyx) index of a world point, as sample_sdfs computes it.
+template <typename T>
+__device__ __forceinline__ void zyx_index(const T* p, const T* mn, const T* mx, const GridDesc& g,
+                                          T ind[3]) {
+#pragma clang fp contract(off)
+  // ind_norm (zyx) then ((ind_norm + 1) / 2) * (size_zyx - 1)
+  const int sz[3] = {g.Z, g.Y, g.X};
+  const int ax[3] = {2, 1, 0};
+  for (int k = 0; k < 3; ++k) {
+    const int a = ax[k];
+    const T n = ((p[a] - mn[a]) / (mx[a] - mn[a])) * (T)2 - (T)1;
+    ind[k] = ((n + (T)1) / (T)2) * (T)(sz[k] - 1);
+  }
+}
+
+// Sample of the single-channel grid at the zyx index with per-component displacement;
+// returns the value and the clamped displaced index along `axis`.
+template <typename T>
+__device__ __forceinline__ T sample_displaced(const T* __restrict__ grid, const GridDesc& g,
+                                              const T ind[3], int axis, T off, T& clamped) {
+#pragma clang fp contract(off)
+  const int sz[3] = {g.Z, g.Y, g.X};
+  T pos[3];
+  for (int k = 0; k < 3; ++k) {
+    T a = ind[k] + (k == axis ? off : (T)0);
+    a = clampT(a, (T)0, (T)(sz[k] - 1));
+    if (k == axis) clamped = a;
+    pos[k] = roundtrip_index(a, sz[k]);
+  }
+  Tri<T> t;
+  t.init(pos[2], pos[1], pos[0]);
+  return tri_gather(grid, g, t, 0);
+}
+
+template <typename T>
+__device__ __forceinline__ void scatter_displaced(T* __restrict__ grad, const GridDesc& g,
+                                                  const T ind[3], int axis, T off, T go) {
+#pragma clang fp contract(off)
+  const int sz[3] = {g.Z, g.Y, g.X};
+  T pos[3];
+  for (int k = 0; k < 3; ++k) {
+    T a = ind[k] + (k == axis ? off : (T)0);
+    a = clampT(a, (T)0, (T)(sz[k] - 1));
+    pos[k] = roundtrip_index(a, sz[k]);
+  }
+  Tri<T> t;
+  t.init(pos[2], pos[1], pos[0]);
+  tri_scatter(grad, g, t, 0, go);
+}
+
+// Clamped displaced index pair along `axis` (no sampling): (ind[+], ind[-]).
+template <typename T>
+__device__ __forceinline__ T clamped_diff(const T ind[3], int axis, T d, const GridDesc& g) {
+#pragma clang fp contract(off)
+  const int sz[3] = {g.Z, g.Y, g.X};
+  const T hi = (T)(sz[axis] - 1);
+  const T ap = clampT(ind[axis] + d, (T)0, hi);
+  const T am = clampT(ind[axis] + (-d), (T)0, hi);
+  // diff.max(dim=-1) over the 3 components: the other two components are 0
+  const T df = ap - am;
+  return df > (T)0 ? df : (T)0;
+}
+
+
+// ---- NeuS alpha (lib/voxurf_fine.py:463-500) ------------------------------------------
+
+template <typename T>
+__device__ __forceinline__ T sigmoidT(T x) {
+  return (T)1 / ((T)1 + exp(-x));
+}
+
+// NeuS alpha, use_mid=True, cos_anneal_ratio=1:
+//   cos  = <viewdir[ray], grad>
+//   ic   = -relu(-cos)
+//   prev = sdf - ic*dist*0.5,  next = sdf + ic*dist*0.5
+//   c    = sigmoid(prev*inv_s), p = c - sigmoid(next*inv_s)
+//   alpha= clip((p + 1e-5) / (c + 1e-5), 0, 1)
+template <typename T>
+struct NeusFwd {
+  T cos, ic, c, nx, a;
+};
+
+template <typename T>
+__device__ __forceinline__ NeusFwd<T> neus_fwd(const T* dir, const T* gr, T sdf, T dist, T inv_s) {
+#pragma clang fp contract(off)
+  NeusFwd<T> f;
+  f.cos = dir[0] * gr[0] + dir[1] * gr[1] + dir[2] * gr[2];
+  const T r = -f.cos > (T)0 ? -f.cos : (T)0;
+  f.ic = -r;
+  const T h = (f.ic * dist) * (T)0.5;
+  const T nxt = sdf + h;
+  const T prv = sdf - h;
+  f.c = sigmoidT(prv * inv_s);
+  f.nx = sigmoidT(nxt * inv_s);
+  const T p = f.c - f.nx;
+  f.a = (p + (T)1e-5) / (f.c + (T)1e-5);
+  return f;
+}
+
+template <typename T>
+__device__ __forceinline__ T clip01(T a) {
+  return a < (T)0 ? (T)0 : (a > (T)1 ? (T)1 : a);
+}
+
+// Adjoint of clip(neus alpha) w.r.t. sdf and the gradient vector: clip passes the
+// gradient on [0, 1] inclusive, relu(x) only for x > 0, sigmoid' = y (1 - y).
+template <typename T>
+__device__ __forceinline__ void neus_bwd(const T* dir, const NeusFwd<T>& f, T dist, T inv_s, T ga,
+                                         T& gsdf, T gg[3]) {
+#pragma clang fp contract(off)
+  if (!(f.a >= (T)0 && f.a <= (T)1)) ga = 0;
+  const T den = f.c + (T)1e-5;
+  const T gp = ga / den;
+  const T gc = -ga * ((f.c - f.nx + (T)1e-5) / (den * den)) + gp;
+  const T gn = -gp;
+  const T gprev = gc * (f.c * ((T)1 - f.c)) * inv_s;
+  const T gnext = gn * (f.nx * ((T)1 - f.nx)) * inv_s;
+  gsdf = gprev + gnext;
+  const T gic = (gnext - gprev) * dist * (T)0.5;
+  const T gcos = (-f.cos > (T)0) ? gic : (T)0;
+  gg[0] = gcos * dir[0];
+  gg[1] = gcos * dir[1];
+  gg[2] = gcos * dir[2];
+}
+
+// ---- centre SDF + 6-neighbour gradient (lib/voxurf_fine.py:502-534, 537-577) ----------
+// sdf at the point, grad[3] in xyz order (unnormalised), optional feat[6]
+// (-x,+x,-y,+y,-z,+z).
+
+template <typename T>
+__device__ __forceinline__ void sdf7(const T* __restrict__ grid, const GridDesc& g, const T q[3],
+                                     const T* mn, const T* mx, T voxel_size, T& sdf, T grad[3],
+                                     T* feat) {
+#pragma clang fp contract(off)
+  Tri<T> t;
+  t.init(world_to_index(q[0], mn[0], mx[0] - mn[0], g.X),
+         world_to_index(q[1], mn[1], mx[1] - mn[1], g.Y),
+         world_to_index(q[2], mn[2], mx[2] - mn[2], g.Z));
+  sdf = tri_gather(grid, g, t, 0);
+  T ind[3];
+  zyx_index(q, mn, mx, g, ind);
+  for (int a = 0; a < 3; ++a) {
+    T cm, cp;
+    const T fm = sample_displaced(grid, g, ind, a, (T)-1, cm);
+    const T fp = sample_displaced(grid, g, ind, a, (T)1, cp);
+    const T df = clamped_diff(ind, a, (T)1, g);
+    grad[2 - a] = ((fp - fm) / df) / voxel_size;
+    if (feat) {
+      feat[2 * (2 - a)] = fm;
+      feat[2 * (2 - a) + 1] = fp;
+    }
+  }
+}
+
+// Adjoint of sdf7: ggrid += d(sdf, grad, feat)^T (gsdf, ggrad, gfeat); gfeat may be null.
+template <typename T>
+__device__ __forceinline__ void sdf7_scatter(T* __restrict__ ggrid, const GridDesc& g,
+                                             const T q[3], const T* mn, const T* mx,
+                                             T voxel_size, T gsdf, const T ggrad[3],
+                                             const T* gfeat) {
+#pragma clang fp contract(off)
+  if (gsdf != (T)0) {
+    Tri<T> t;
+    t.init(world_to_index(q[0], mn[0], mx[0] - mn[0], g.X),
+           world_to_index(q[1], mn[1], mx[1] - mn[1], g.Y),
+           world_to_index(q[2], mn[2], mx[2] - mn[2], g.Z));
+    tri_scatter(ggrid, g, t, 0, gsdf);
+  }
+  T ind[3];
+  zyx_index(q, mn, mx, g, ind);
+  for (int a = 0; a < 3; ++a) {
+    const int c = 2 - a;
+    const T df = clamped_diff(ind, a, (T)1, g);
+    const T gd = (ggrad[c] / df) / voxel_size;
+    const T gm = (gfeat ? gfeat[2 * c] : (T)0) - gd;
+    const T gp = (gfeat ? gfeat[2 * c + 1] : (T)0) + gd;
+    scatter_displaced(ggrid, g, ind, a, (T)-1, gm);
+    scatter_displaced(ggrid, g, ind, a, (T)1, gp);
+  }
+}
+
+
+}  // namespace vx

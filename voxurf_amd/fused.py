@@ -1,0 +1,171 @@
+"""The fine training step on the fused per-ray HIP kernels.
+
+One step (run.py:600-659 for the fine model, lib/voxurf_fine.py:620-802):
+
+  1. vx_fused_fine_fwd     one wavefront per ray: sampling, SDF + 6-neighbour gradient,
+                           NeuS alpha, the transmittance walk; survivors (w > thres)
+                           into per-ray slots
+  2. scan of survivor counts -> row offsets; the total is read back once (the
+                           reference reads sample totals back several times per step)
+  3. vx_fused_fine_gather  compaction + rgbnet input rows [k0 | normal | view PE]
+  4. rgbnet forward        (voxurf_amd.mlp_hip)
+  5. vx_fused_fine_composite  compositing, MSE loss, colour-side adjoints
+  6. rgbnet backward
+  7. vx_fused_fine_feat_bwd   k0 scatter, normal adjoint
+  8. vx_fused_fine_bwd        per-ray recomputing backward of the SDF side
+  9. Adam (utils.Adam semantics) over sdf, k0 and rgbnet, clearing the grid gradients
+
+Grids: sdf [1,1,R,R,R] contiguous; k0 [1,C,R,R,R] channels-last.  Gradient buffers
+persist across steps (cleared by the Adam pass instead of reallocated).
+"""
+import math
+
+import torch
+
+from . import _lib
+from ._lib import call, ptr, stream_of
+from .mlp import RgbNet
+from .optim import Adam
+from .total_variation_cuda import grid_geom
+
+XSTRIDE = 48
+
+
+class FusedFineTrainer:
+    def __init__(self, cfg, sdf, k0, mlp_params=None, device="cuda:0", mlp_dtype=torch.float32,
+                 lr_scale=1.0):
+        self.cfg = cfg
+        self.dev = torch.device(device)
+        self.sdf = sdf.to(self.dev, torch.float32).contiguous()
+        self.k0 = k0.to(self.dev, torch.float32).contiguous(memory_format=torch.channels_last_3d)
+        self.sdf_grad = torch.zeros_like(self.sdf)
+        self.k0_grad = torch.zeros_like(self.k0)
+        self.adam = {
+            "sdf": [torch.zeros_like(self.sdf), torch.zeros_like(self.sdf), 0, cfg.lr_sdf],
+            "k0": [torch.zeros_like(self.k0), torch.zeros_like(self.k0), 0, cfg.lr_k0],
+        }
+        self.rgbnet = RgbNet(cfg.mlp_in, cfg.mlp_width, cfg.mlp_depth, params=mlp_params,
+                             compute_dtype=mlp_dtype).to(self.dev)
+        self.mlp_opt = Adam(self.rgbnet.parameters(), lr=cfg.lr_mlp, betas=(0.9, 0.99))
+        self.sdf_geom = grid_geom(self.sdf)
+        self.k0_geom = grid_geom(self.k0)
+        self.box = torch.tensor([-1.0, -1.0, -1.0, 1.0, 1.0, 1.0], dtype=torch.float32)
+        self.params = torch.tensor([cfg.stepdist, cfg.voxel_size, cfg.inv_s, cfg.fast_color_thres,
+                                    0.0, 1e9], dtype=torch.float32)
+        diag = math.sqrt(3.0) * 2.0
+        self.cap = int(math.ceil(diag / cfg.stepdist)) + 2
+        self._n = -1
+        self.last_profile = {}
+        self.last_counts = {}
+        self.count_all = True
+
+    # --- buffers ------------------------------------------------------------------
+    def _ensure(self, n):
+        if n == self._n:
+            return
+        d = self.dev
+        self.s4 = torch.empty((n * self.cap, 4), dtype=torch.float32, device=d)
+        self.g4 = torch.empty((n * self.cap, 4), dtype=torch.float32, device=d)
+        self.cnt = torch.empty(n, dtype=torch.int64, device=d)
+        self.off = torch.empty(n, dtype=torch.int64, device=d)
+        self.total = torch.empty(1, dtype=torch.int64, device=d)
+        self.ws = torch.empty(max(1, _lib.lib().vx_scan_workspace_elems(n)), dtype=torch.int64, device=d)
+        self.last = torch.empty(n, dtype=torch.float32, device=d)
+        self.stats = torch.empty((n, 3), dtype=torch.int32, device=d)
+        self.overflow = torch.zeros(1, dtype=torch.int32, device=d)
+        self.rgb = torch.empty((n, 3), dtype=torch.float32, device=d)
+        self.glast = torch.empty(n, dtype=torch.float32, device=d)
+        self.S = torch.empty(n, dtype=torch.float32, device=d)
+        self.loss_r = torch.empty(n, dtype=torch.float32, device=d)
+        self.loss = torch.empty(1, dtype=torch.float32, device=d)
+        self._n = n
+
+    def _adam(self, name, p, g, profile=False):
+        m, v, step, lr = self.adam[name]
+        step += 1
+        self.adam[name][2] = step
+        st = stream_of(p)
+        if profile:
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+        call("vx_adam_step", 0, ptr(p), ptr(g), ptr(m), ptr(v), None, p.numel(), step, float(lr),
+             0.9, 0.99, 1e-8, 1, st)
+        if profile:
+            e1.record()
+            return (e0, e1)
+        return None
+
+    # --- one step -----------------------------------------------------------------
+    def forward(self, rays_o, rays_d, viewdirs):
+        """Steps 1-3: survivors and rgbnet input rows; returns M (survivor count)."""
+        n = rays_o.shape[0]
+        self._ensure(n)
+        st = stream_of(self.sdf)
+        call("vx_fused_fine_fwd", ptr(self.sdf), self.sdf_geom.data_ptr(), self.box.data_ptr(),
+             self.params.data_ptr(), ptr(rays_o), ptr(rays_d), ptr(viewdirs), n, self.cap,
+             ptr(self.s4), ptr(self.g4), ptr(self.cnt), ptr(self.last), ptr(self.stats),
+             int(self.count_all), ptr(self.overflow), st)
+        call("vx_inclusive_scan_i64", ptr(self.cnt), ptr(self.off), n, ptr(self.ws), st)
+        m = int(self.off[-1].item()) if n > 0 else 0
+        self.m = m
+        self.X = torch.empty((max(m, 1), XSTRIDE), dtype=torch.float32, device=self.dev)
+        self.P4 = torch.empty((max(m, 1), 4), dtype=torch.float32, device=self.dev)
+        self.G4 = torch.empty((max(m, 1), 4), dtype=torch.float32, device=self.dev)
+        self.rid = torch.empty(max(m, 1), dtype=torch.int32, device=self.dev)
+        call("vx_fused_fine_gather", ptr(self.k0), self.k0_geom.data_ptr(),
+             self.sdf_geom.data_ptr(), self.box.data_ptr(), self.params.data_ptr(),
+             ptr(viewdirs), n, self.cap, self.cfg.view_pe, ptr(self.s4), ptr(self.g4),
+             ptr(self.cnt), ptr(self.off), XSTRIDE, ptr(self.X), ptr(self.P4), ptr(self.G4),
+             ptr(self.rid), st)
+        return m
+
+    def step(self, rays_o, rays_d, viewdirs, target, profile=False):
+        cfg = self.cfg
+        n = rays_o.shape[0]
+        m = self.forward(rays_o, rays_d, viewdirs)
+        st = stream_of(self.sdf)
+        din = cfg.mlp_in
+        xin = self.X[:m, :din].detach().requires_grad_(True)
+        logits = self.rgbnet(xin)
+        lg = logits.detach().contiguous()
+        dlogit = torch.empty((max(m, 1), 3), dtype=torch.float32, device=self.dev)
+        gw = torch.empty(max(m, 1), dtype=torch.float32, device=self.dev)
+        call("vx_fused_fine_composite", n, ptr(self.cnt), ptr(self.off), ptr(lg), 3, ptr(self.P4),
+             ptr(self.last), ptr(target), float(cfg.bg), ptr(self.rgb), ptr(dlogit), ptr(gw),
+             ptr(self.glast), ptr(self.S), ptr(self.loss_r), st)
+        for p in self.rgbnet.parameters():
+            p.grad = None
+        if m > 0:
+            logits.backward(dlogit[:m])
+            dX = xin.grad
+            assert dX.stride(1) == 1
+        else:
+            dX = torch.zeros((1, din), dtype=torch.float32, device=self.dev)
+        dgf = torch.empty((max(m, 1), 3), dtype=torch.float32, device=self.dev)
+        call("vx_fused_fine_feat_bwd", self.k0_geom.data_ptr(), self.sdf_geom.data_ptr(),
+             self.box.data_ptr(), self.params.data_ptr(), m, ptr(dX), dX.stride(0), ptr(self.P4),
+             ptr(self.G4), ptr(self.k0_grad), ptr(dgf), st)
+        call("vx_fused_fine_bwd", ptr(self.sdf), self.sdf_geom.data_ptr(), self.box.data_ptr(),
+             self.params.data_ptr(), ptr(rays_o), ptr(rays_d), ptr(viewdirs), n, ptr(self.cnt),
+             ptr(self.off), ptr(self.last), ptr(gw), ptr(self.glast), ptr(self.S), ptr(dgf),
+             ptr(self.sdf_grad), st)
+        if profile:
+            self.grads_snapshot = dict(sdf=self.sdf_grad.clone(), k0=self.k0_grad.clone(),
+                                       mlp=[p.grad.clone() for p in self.rgbnet.parameters()])
+        self._adam("sdf", self.sdf, self.sdf_grad)
+        ev = self._adam("k0", self.k0, self.k0_grad, profile=profile)
+        self.mlp_opt.step()
+        call("vx_sum_f32", ptr(self.loss_r), n, ptr(self.loss), st)
+        if profile:
+            torch.cuda.synchronize()
+            self.last_profile = {"adam_k0_ms": ev[0].elapsed_time(ev[1])}
+            s = self.stats[:n].sum(0).tolist()
+            self.last_counts = {"m0": int(s[2]), "m0_processed": int(s[0]), "m1b": int(s[1]),
+                                "m2": m}
+        return dict(loss=self.loss / (3.0 * n), rgb_marched=self.rgb[:n], n_samples=m)
+
+    def state_dict(self):
+        sd = {"sdf.grid": self.sdf, "k0.grid": self.k0}
+        for k, v in self.rgbnet.state_dict().items():
+            sd["rgbnet." + k] = v
+        return sd

@@ -1,0 +1,164 @@
+"""Voxurf coarse/fine forward passes and training steps on the HIP ops.
+
+VoxurfFine / VoxurfCoarse keep the reference models' parameter names (sdf.grid, k0.grid,
+rgbnet.*), buffers (xyz_min, xyz_max, viewfreq) and call structure
+(lib/voxurf_fine.py:620-802, lib/voxurf_coarse.py:513-619): ray sampling ->
+in-bbox compaction -> SDF value and 6-neighbour gradient -> NeuS alpha ->
+alpha/weight compactions -> Alphas2Weights -> feature lookup -> rgbnet ->
+segment-sum compositing.  Every step of it runs on the HIP kernels of libvoxhip.so
+through the autograd Functions of voxurf_amd.functional; this is the "store-all" path
+(every sample's intermediates are kept for backward, as the reference does).  The fused
+per-ray path lives in voxurf_amd.fused.
+"""
+import torch
+import torch.nn as nn
+import torch.nn.functional as F
+
+from . import render_utils_cuda as ru
+from .functional import (Alphas2Weights, NeusAlpha, SdfGradSample, segment_coo)
+from .grid import DenseGrid
+from .mlp import RgbNet
+from .scene import SceneConfig
+
+
+class _VoxurfBase(nn.Module):
+    def __init__(self, cfg: SceneConfig, sdf_grid=None, k0_grid=None):
+        super().__init__()
+        self.cfg = cfg
+        R = cfg.R
+        self.register_buffer("xyz_min", torch.tensor([-1.0, -1.0, -1.0]))
+        self.register_buffer("xyz_max", torch.tensor([1.0, 1.0, 1.0]))
+        self.world_size = torch.tensor([R, R, R])
+        self.voxel_size = cfg.voxel_size
+        self.sdf = DenseGrid(1, [R, R, R], [-1.0] * 3, [1.0] * 3)
+        self.k0 = DenseGrid(cfg.feat_dim, [R, R, R], [-1.0] * 3, [1.0] * 3)
+        if sdf_grid is not None:
+            self.sdf.grid.data.copy_(sdf_grid)
+        if k0_grid is not None:
+            self.k0.grid.data.copy_(k0_grid)
+        self.s_val = nn.Parameter(torch.ones(1) / cfg.inv_s, requires_grad=False)
+
+    @property
+    def inv_s(self):
+        return self.cfg.inv_s
+
+    def sample_ray(self, rays_o, rays_d, stepsize):
+        """lib/voxurf_fine.py:593-617 (far forced to 1e9)."""
+        stepdist = self.cfg.stepdist
+        ray_pts, mask_outbbox, ray_id, step_id, N_steps, t_min, t_max = ru.sample_pts_on_rays(
+            rays_o.contiguous(), rays_d.contiguous(), self.xyz_min, self.xyz_max, 0.0, 1e9,
+            stepdist)
+        mask_inbbox = ~mask_outbbox
+        return ray_pts[mask_inbbox], ray_id[mask_inbbox], step_id[mask_inbbox], mask_outbbox, N_steps
+
+    def sdf_and_gradient(self, ray_pts):
+        sdf, gradient, _ = SdfGradSample.apply(self.sdf.grid, ray_pts, self.xyz_min, self.xyz_max,
+                                                self.voxel_size)
+        return sdf, gradient
+
+    def total_variation_add_grad(self, weight, dense_mode=True):
+        self.sdf.total_variation_add_grad(weight, weight, weight, dense_mode)
+        self.k0.total_variation_add_grad(weight, weight, weight, dense_mode)
+
+
+class VoxurfFine(_VoxurfBase):
+    """Fine stage: 12-channel k0, rgbnet(k0 | normal | view PE) (lib/voxurf_fine.py)."""
+
+    def __init__(self, cfg: SceneConfig, sdf_grid=None, k0_grid=None, mlp_params=None,
+                 mlp_dtype=torch.float32):
+        super().__init__(cfg, sdf_grid, k0_grid)
+        self.register_buffer("viewfreq", torch.tensor([2.0 ** i for i in range(cfg.view_pe)]))
+        self.rgbnet = RgbNet(cfg.mlp_in, cfg.mlp_width, cfg.mlp_depth, params=mlp_params,
+                             compute_dtype=mlp_dtype)
+
+    def forward(self, rays_o, rays_d, viewdirs, bg=0.0):
+        cfg = self.cfg
+        N = len(rays_o)
+        ray_pts, ray_id, step_id, mask_outbbox, N_steps = self.sample_ray(rays_o, rays_d,
+                                                                          cfg.stepsize)
+        sdf, gradient = self.sdf_and_gradient(ray_pts)
+        alpha = NeusAlpha.apply(viewdirs, ray_id, sdf, gradient, cfg.stepdist, self.inv_s)
+        m = alpha > cfg.fast_color_thres
+        alpha, ray_id, ray_pts, gradient, step_id = alpha[m], ray_id[m], ray_pts[m], gradient[m], step_id[m]
+        weights, alphainv_last = Alphas2Weights.apply(alpha, ray_id, N)
+        m = weights > cfg.fast_color_thres
+        weights, ray_id, ray_pts, gradient, step_id = (weights[m], ray_id[m], ray_pts[m],
+                                                       gradient[m], step_id[m])
+        k0 = self.k0(ray_pts)
+        normal = gradient / (gradient.norm(dim=-1, keepdim=True) + 1e-5)
+        vemb = (viewdirs.unsqueeze(-1) * self.viewfreq).flatten(-2)
+        vemb = torch.cat([viewdirs, vemb.sin(), vemb.cos()], -1)
+        rgb_feat = torch.cat([k0, normal, vemb[ray_id]], -1)
+        rgb = torch.sigmoid(self.rgbnet(rgb_feat))
+        rgb_marched = segment_coo(weights.unsqueeze(-1) * rgb, ray_id,
+                                  out=torch.zeros([N, 3], device=rgb.device), reduce="sum")
+        rgb_marched = rgb_marched + alphainv_last[..., None] * bg
+        return dict(rgb_marched=rgb_marched, alphainv_cum=alphainv_last, weights=weights,
+                    ray_id=ray_id, n_samples=int(ray_id.numel()))
+
+
+class VoxurfCoarse(_VoxurfBase):
+    """Coarse stage: colour = sigmoid(k0) with a 3-channel k0 (lib/voxurf_coarse.py)."""
+
+    def forward(self, rays_o, rays_d, viewdirs, bg=0.0):
+        cfg = self.cfg
+        N = len(rays_o)
+        ray_pts, ray_id, step_id, mask_outbbox, N_steps = self.sample_ray(rays_o, rays_d,
+                                                                          cfg.stepsize)
+        sdf, gradient = self.sdf_and_gradient(ray_pts)
+        alpha = NeusAlpha.apply(viewdirs, ray_id, sdf, gradient, cfg.stepdist, self.inv_s)
+        weights, alphainv_last = Alphas2Weights.apply(alpha, ray_id, N)
+        m = weights > cfg.fast_color_thres
+        ray_pts, ray_id, alpha = ray_pts[m], ray_id[m], alpha[m]
+        weights, alphainv_last = Alphas2Weights.apply(alpha, ray_id, N)
+        rgb = torch.sigmoid(self.k0(ray_pts))
+        rgb_marched = segment_coo(weights.unsqueeze(-1) * rgb, ray_id,
+                                  out=torch.zeros([N, 3], device=rgb.device), reduce="sum")
+        cum_w = segment_coo(weights.unsqueeze(-1), ray_id,
+                            out=torch.zeros([N, 1], device=rgb.device), reduce="sum")
+        rgb_marched = (rgb_marched + (1 - cum_w) * bg).clamp(0, 1)
+        return dict(rgb_marched=rgb_marched, alphainv_cum=alphainv_last, weights=weights,
+                    ray_id=ray_id, n_samples=int(ray_id.numel()))
+
+
+class Trainer:
+    """The training step of run.py:600-659 around a VoxurfFine/VoxurfCoarse model:
+    forward, MSE loss, backward, TV gradient (if cfg.tv_weight > 0), Adam
+    (utils.Adam, betas (0.9, 0.99)) with per-module learning rates."""
+
+    def __init__(self, model, cfg: SceneConfig):
+        from .optim import Adam
+        self.model = model
+        self.cfg = cfg
+        groups = [dict(params=[model.sdf.grid], lr=cfg.lr_sdf, name="sdf"),
+                  dict(params=[model.k0.grid], lr=cfg.lr_k0, name="k0")]
+        if hasattr(model, "rgbnet") and cfg.lr_mlp > 0:
+            groups.append(dict(params=list(model.rgbnet.parameters()), lr=cfg.lr_mlp,
+                               name="rgbnet"))
+        self.opt = Adam(groups, betas=(0.9, 0.99))
+
+        self.last_profile = {}
+        self.last_counts = {}
+
+    def step(self, rays_o, rays_d, viewdirs, target, profile=False):
+        self.opt.zero_grad(set_to_none=True)
+        out = self.model(rays_o, rays_d, viewdirs, bg=self.cfg.bg)
+        loss = F.mse_loss(out["rgb_marched"], target)
+        loss.backward()
+        if self.cfg.tv_weight > 0:
+            self.model.total_variation_add_grad(self.cfg.tv_weight, True)
+        if profile:
+            self.grads_snapshot = dict(
+                sdf=self.model.sdf.grid.grad.clone(), k0=self.model.k0.grid.grad.clone(),
+                mlp=[p.grad.clone() for p in self.model.rgbnet.parameters()]
+                if hasattr(self.model, "rgbnet") else [])
+            self.opt.profile_events = {}
+        self.opt.step()
+        if profile:
+            torch.cuda.synchronize()
+            e0, e1 = self.opt.profile_events[id(self.model.k0.grid)]
+            self.last_profile = {"adam_k0_ms": e0.elapsed_time(e1)}
+            self.opt.profile_events = None
+            self.last_counts = {"m2": out["n_samples"]}
+        out["loss"] = loss.detach()
+        return out
