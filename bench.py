@@ -43,6 +43,8 @@ def step_bytes(cfg, counts):
     scatters), from the per-step sample counts."""
     R, C = cfg.R, cfg.feat_dim
     n_e = R ** 3 * (1 + C)
+    if not all(k in counts for k in ("m0", "m1b", "m2")):
+        return None
     m0, m1b, m2 = counts["m0"], counts["m1b"], counts["m2"]
     return (32 * n_e + 60 * cfg.n_rays + m0 * 7 * 8 * 4 + m2 * 8 * C * 4 + m1b * 7 * 8 * 8
             + m2 * 8 * C * 8)
@@ -152,8 +154,9 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "adam_step (k0 grid, 201M fp32 elements)",
                      "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": None},
-        "step_roofline": {"algorithmic_bytes": sb, "achieved_GBs": sb / (ms * 1e-3) / 1e9,
-                          "frac": sb / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS},
+        "step_roofline": None if sb is None else {
+            "algorithmic_bytes": sb, "achieved_GBs": sb / (ms * 1e-3) / 1e9,
+            "frac": sb / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS},
         "samples": counts,
         "wall_s": wall,
     }

@@ -255,6 +255,42 @@ int vx_fused_fine_bwd(const float* sdf, const int64_t* sdf_geom, const float* bo
                       const int64_t* off_incl, const float* last, const float* gw,
                       const float* glast, const float* S, const float* dgf, float* sdf_grad,
                       void* stream);
+/* ---- rgbnet on the matrix cores ---------------------------------------------------
+ * The colour MLP of lib/voxurf_fine.py:132-149 (nn.Linear/ReLU layers of width 192,
+ * which the reference runs as cuBLAS GEMMs plus elementwise kernels).  bf16 = 0: FP32
+ * MFMAs; bf16 = 1: BF16 MFMAs with FP32 accumulation and BF16 activations.
+ * Weights are packed per call into the MFMA operand order; activation rows are
+ * [M, 192] (element type per bf16), inputs [M, 32*kt] zero padded. */
+
+/* Pack W [n_out, k_in] (row-major, float32) as the A operand of a layer with kt input
+ * tiles of 32 (reduction) and nto output tiles; transpose = 1 packs W^T (backward). */
+int vx_mlp_pack(int bf16, const float* W, int n_out, int k_in, int kt, int nto, int transpose,
+                void* out, void* stream);
+/* Element count of a packed operand. */
+int64_t vx_mlp_pack_elems(int kt, int nto);
+/* One hidden layer: out = relu(W in + b); with last = 1 also logits [M,3] = W3 out + b3. */
+int vx_mlp_layer_fwd(int bf16, int kt, int last, const void* in, int in_stride, int64_t M,
+                     const void* wpk, const float* bias, void* out, int out_stride,
+                     const float* w3, const float* b3, float* logits, void* stream);
+/* Data gradient through a layer: dx = (W^T dy) * [mask > 0] (mask NULL: no ReLU). */
+int vx_mlp_layer_bwd(int bf16, int nto, const float* dy, int dy_stride, int64_t M,
+                     const void* wpk_t, const void* mask, int mask_stride, float* dx,
+                     int dx_stride, void* stream);
+/* dH = (dlogit W3) * [H > 0] for the 3-output layer. */
+int vx_mlp_dh_last(int bf16, const float* dlogit, const void* H, int h_stride, int64_t M,
+                   const float* w3, float* dH, int dh_stride, void* stream);
+/* Row blocks of the weight-gradient kernels for M rows (partial buffers hold that many
+ * [192 x 32*kt] slices). */
+int vx_mlp_wgrad_blocks(int64_t M);
+/* dW [192, k_real] = dY^T X, db [192] = column sums of dY (split over row blocks). */
+int vx_mlp_wgrad(int bf16, int kt, int k_real, const float* dY, int dys, const void* X, int xs,
+                 int64_t M, float* part, float* partb, float* dW, float* db, void* stream);
+/* dW3 [3, 192] = dlogit^T H, db3 [3] = column sums of dlogit. */
+int vx_mlp_wgrad_last(int bf16, const float* dlogit, const void* H, int hs, int64_t M,
+                      float* part, float* partb, float* dW3, float* db3, void* stream);
+/* y = bfloat16(x), n elements. */
+int vx_f32_to_bf16(const float* x, int64_t n, void* y, void* stream);
+
 /* *out = sum of x[0..n) (float32, device scalar). */
 int vx_sum_f32(const float* x, int64_t n, float* out, void* stream);
 

@@ -91,11 +91,24 @@ def test_fine_step_matches_restatement(dev, setup, path):
         names = [f"mlp{i // 2}.{'w' if i % 2 == 0 else 'b'}" for i in range(len(grads["mlp"]))]
         for g, nm in zip(grads["mlp"], names):
             close(g, ref["grads"][nm], 1e-4, 1e-4, nm + " grad")
-        close(sdf_after, ref_after["sdf"], 1e-5, 1e-6, "sdf after Adam")
-        close(k0_after, ref_after["k0"], 1e-5, 1e-6, "k0 after Adam")
-        flat = [x for wb in ref_after["mlp"] for x in wb]
-        for p, q in zip(mlp_after, flat):
-            close(p, q, 1e-5, 1e-6, "rgbnet after Adam")
+        # Adam at step 1 moves each element by about -lr * g / (|g| + eps), so elements
+        # whose gradient is within rounding noise of zero may move differently from the
+        # restatement.  The optimiser is therefore checked against utils.Adam applied
+        # to this path's own gradients (1e-5), and against the restatement's parameters
+        # only where the gradient is well above its noise floor.
+        pairs = [("sdf", st["sdf"], grads["sdf"], sdf_after, CFG.lr_sdf, ref_after["sdf"], ref["grads"]["sdf"]),
+                 ("k0", st["k0"], grads["k0"], k0_after, CFG.lr_k0, ref_after["k0"], ref["grads"]["k0"])]
+        flat0 = [x for wb in st["mlp"] for x in wb]
+        flat1 = [x for wb in ref_after["mlp"] for x in wb]
+        for i, (p0, g, p1, q1) in enumerate(zip(flat0, grads["mlp"], mlp_after, flat1)):
+            pairs.append((f"rgbnet[{i}]", p0, g, p1, CFG.lr_mlp, q1, ref["grads"][names[i]]))
+        for name, p0, g, p1, lr, q1, gref in pairs:
+            exp = p0.clone().double()
+            tr.adam_step(exp, g.detach().cpu().double(), torch.zeros_like(exp), torch.zeros_like(exp),
+                         1, lr, 0.9, 0.99, 1e-8)
+            close(p1, exp, 1e-5, 1e-7, name + " after Adam (own gradient)")
+            big = gref.abs() > 1e-3 * float(gref.abs().max())
+            close(p1.detach().cpu()[big.cpu()], q1[big], 1e-5, 1e-6, name + " after Adam vs restatement")
 
 
 def test_fused_equals_ops_survivors_exactly(dev, setup):
