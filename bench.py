@@ -34,6 +34,16 @@ def parse():
     ap.add_argument("--rays", type=int, default=8192)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--mode", choices=["train", "coarse", "recompute", "precision", "serve"],
+                    default="train",
+                    help="train: the plain fine step (the headline line); coarse: the plain "
+                         "coarse step; recompute: 65536-ray fine step, per-ray recomputation "
+                         "vs store-all; precision: BF16 rgbnet vs FP32 vs FP64; serve: the "
+                         "render/SDF-query server vs sequential per-request rendering")
+    ap.add_argument("--requests", type=int, default=300)
+    ap.add_argument("--rate", type=float, default=50.0)
+    ap.add_argument("--seeds", type=int, default=10)
+    ap.add_argument("--psnr-steps", type=int, default=200)
     return ap.parse_args()
 
 
@@ -74,6 +84,11 @@ def main():
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 "
                                    "(multi-GPU sharding is a design note only)"}))
         sys.exit(2)
+    if args.mode != "train":
+        from voxurf_amd import studies
+        res = getattr(studies, args.mode)(args)
+        print(json.dumps(res))
+        return
     import torch
     from voxurf_amd import scene
     from voxurf_amd.fused import FusedFineTrainer

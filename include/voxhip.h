@@ -236,7 +236,8 @@ int vx_fused_fine_gather(const float* k0, const int64_t* k0_geom, const int64_t*
                          void* P4, void* G4, int* rid, void* stream);
 /* segment_coo compositing (lib/voxurf_fine.py:753-763), MSE loss per ray (run.py:604)
  * and the colour-side adjoints: dlogit [M,3], gw [M] (d loss / d weight), glast [N],
- * S [N] (sum of gw*w per ray), loss_r [N] (sum of squared errors). */
+ * S [N] (sum of gw*w per ray), loss_r [N] (sum of squared errors).  With target NULL
+ * only rgb [N,3] is written (rendering; the adjoint pointers may be NULL). */
 int vx_fused_fine_composite(int64_t n_rays, const int64_t* cnt, const int64_t* off_incl,
                             const float* logits, int lstride, const void* P4, const float* last,
                             const float* target, float bg, float* rgb, float* dlogit, float* gw,

@@ -235,6 +235,10 @@ __global__ void k_fused_composite(int64_t n_rays, const int64_t* __restrict__ cn
       }
     }
     float g[3], l = 0.f;
+    if (!target) {  // rendering only
+      for (int ch = 0; ch < 3; ++ch) rgb_out[3 * r + ch] = acc[ch] + last[r] * bg;
+      continue;
+    }
     for (int ch = 0; ch < 3; ++ch) {
       const float v = acc[ch] + last[r] * bg;
       rgb_out[3 * r + ch] = v;

@@ -34,5 +34,9 @@ class RgbNet(nn.Sequential):
         return out
 
     def forward(self, x):
+        if self.compute_dtype == torch.float64:
+            # float64 evaluation (the error reference of the precision study): the MFMA
+            # kernels are float32/bfloat16 only, so this runs torch's own Linear layers.
+            return super().forward(x)
         from .mlp_hip import mlp_forward
         return mlp_forward(x, self.linears(), self.compute_dtype)

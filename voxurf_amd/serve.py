@@ -1,0 +1,347 @@
+"""Render and SDF-query serving over scenes resident in HBM, with continuous batching.
+
+The reference renders one view at a time, in 8192-ray chunks, one chunk after another
+(run.py:81-227, chunking at :123-126).  Here:
+
+  * Scenes (SDF grid, channels-last feature grid, rgbnet) stay resident on the device.
+  * Requests are renders (rays -> rgb, the fused per-ray forward + rgbnet) or SDF /
+    normal queries on a lattice tile (the lattice query of lib/dvgo_ori.py:679-693 with
+    the 6-neighbour normal of lib/voxurf_fine.py:502-534).
+  * A scheduler admits queued requests while their output buffers fit the HBM budget,
+    and packs work items of up to `batch_rays` rays from any admitted requests of one
+    scene (oldest first), so requests that arrive while others run join the next
+    batches instead of waiting for the queue to drain.
+  * Several worker threads, each with its own HIP stream and workspace, run work items
+    concurrently, so batches of different requests/scenes overlap on the device.
+
+Every ray is rendered independently (no batch-dependent arithmetic), so a request's
+result is the same alone and in any mix; the tests check that bit for bit.
+"""
+import dataclasses
+import math
+import queue
+import threading
+import time
+
+import torch
+
+from . import _lib
+from ._lib import call, ptr, stream_of
+from .fused import XSTRIDE
+from .mlp import RgbNet
+from .total_variation_cuda import grid_geom
+
+
+class Workspace:
+    """Per-stream buffers for the fused forward of up to `n` rays."""
+
+    def __init__(self, device, cap):
+        self.dev = device
+        self.cap = cap
+        self.n = -1
+        self.stream = torch.cuda.Stream(device=device)
+
+    def ensure(self, n):
+        if n <= self.n:
+            return
+        d = self.dev
+        self.s4 = torch.empty((n * self.cap, 4), dtype=torch.float32, device=d)
+        self.g4 = torch.empty((n * self.cap, 4), dtype=torch.float32, device=d)
+        self.cnt = torch.empty(n, dtype=torch.int64, device=d)
+        self.off = torch.empty(n, dtype=torch.int64, device=d)
+        self.ws = torch.empty(max(1, _lib.lib().vx_scan_workspace_elems(n)), dtype=torch.int64, device=d)
+        self.last = torch.empty(n, dtype=torch.float32, device=d)
+        self.stats = torch.empty((n, 3), dtype=torch.int32, device=d)
+        self.overflow = torch.zeros(1, dtype=torch.int32, device=d)
+        self.n = n
+
+    def bytes(self):
+        return 0 if self.n < 0 else self.n * (self.cap * 32 + 64)
+
+
+class ResidentScene:
+    """One fitted scene held on the device (fine-stage layout)."""
+
+    def __init__(self, cfg, sdf, k0, mlp_params, device="cuda:0", mlp_dtype=torch.float32):
+        self.cfg = cfg
+        self.dev = torch.device(device)
+        self.sdf = sdf.to(self.dev, torch.float32).contiguous()
+        self.k0 = k0.to(self.dev, torch.float32).contiguous(memory_format=torch.channels_last_3d)
+        self.rgbnet = RgbNet(cfg.mlp_in, cfg.mlp_width, cfg.mlp_depth, params=mlp_params,
+                             compute_dtype=mlp_dtype).to(self.dev)
+        for p in self.rgbnet.parameters():
+            p.requires_grad_(False)
+        self.sdf_geom = grid_geom(self.sdf)
+        self.k0_geom = grid_geom(self.k0)
+        self.box = torch.tensor([-1.0, -1.0, -1.0, 1.0, 1.0, 1.0], dtype=torch.float32)
+        self.params = torch.tensor([cfg.stepdist, cfg.voxel_size, cfg.inv_s, cfg.fast_color_thres,
+                                    0.0, 1e9], dtype=torch.float32)
+        self.cap = int(math.ceil(math.sqrt(3.0) * 2.0 / cfg.stepdist)) + 2
+        self.xyz_min = torch.tensor([-1.0, -1.0, -1.0], device=self.dev)
+        self.xyz_max = torch.tensor([1.0, 1.0, 1.0], device=self.dev)
+
+    def nbytes(self):
+        n = self.sdf.numel() + self.k0.numel() + sum(p.numel() for p in self.rgbnet.parameters())
+        return 4 * n
+
+    @torch.no_grad()
+    def render(self, rays_o, rays_d, viewdirs, ws, bg=0.0):
+        """rgb [N, 3] for rays on the device (fused forward + rgbnet + compositing),
+        launched on torch's current stream."""
+        n = rays_o.shape[0]
+        out = torch.empty((n, 3), dtype=torch.float32, device=self.dev)
+        if n == 0:
+            return out
+        ws.ensure(n)
+        st = stream_of(rays_o)
+        call("vx_fused_fine_fwd", ptr(self.sdf), self.sdf_geom.data_ptr(), self.box.data_ptr(),
+             self.params.data_ptr(), ptr(rays_o), ptr(rays_d), ptr(viewdirs), n, self.cap,
+             ptr(ws.s4), ptr(ws.g4), ptr(ws.cnt), ptr(ws.last), ptr(ws.stats), 0, ptr(ws.overflow), st)
+        call("vx_inclusive_scan_i64", ptr(ws.cnt), ptr(ws.off), n, ptr(ws.ws), st)
+        m = int(ws.off[n - 1].item())
+        X = torch.empty((max(m, 1), XSTRIDE), dtype=torch.float32, device=self.dev)
+        P4 = torch.empty((max(m, 1), 4), dtype=torch.float32, device=self.dev)
+        G4 = torch.empty((max(m, 1), 4), dtype=torch.float32, device=self.dev)
+        rid = torch.empty(max(m, 1), dtype=torch.int32, device=self.dev)
+        call("vx_fused_fine_gather", ptr(self.k0), self.k0_geom.data_ptr(), self.sdf_geom.data_ptr(),
+             self.box.data_ptr(), self.params.data_ptr(), ptr(viewdirs), n, self.cap, self.cfg.view_pe,
+             ptr(ws.s4), ptr(ws.g4), ptr(ws.cnt), ptr(ws.off), XSTRIDE, ptr(X), ptr(P4), ptr(G4),
+             ptr(rid), st)
+        if m > 0:
+            logits = self.rgbnet(X[:m, :self.cfg.mlp_in]).contiguous()
+        else:
+            logits = torch.zeros((1, 3), dtype=torch.float32, device=self.dev)
+        call("vx_fused_fine_composite", n, ptr(ws.cnt), ptr(ws.off), ptr(logits), 3, ptr(P4),
+             ptr(ws.last), None, float(bg), ptr(out), None, None, None, None, None, st)
+        return out
+
+    @torch.no_grad()
+    def sdf_query(self, res, tile_origin, tile=64):
+        """SDF and unit normal at a tile of the res^3 lattice over the bbox
+        (x_i = -1 + 2 i / (res - 1)): points [tile^3, 3] -> (sdf [P], normal [P, 3])."""
+        i0, j0, k0 = tile_origin
+        ax = [(-1.0 + 2.0 * (torch.arange(tile, device=self.dev, dtype=torch.float32) + o) / (res - 1))
+              for o in (i0, j0, k0)]
+        X, Y, Z = torch.meshgrid(*ax, indexing="ij")
+        pts = torch.stack([X, Y, Z], -1).reshape(-1, 3).contiguous()
+        n = pts.shape[0]
+        sdf = torch.empty(n, dtype=torch.float32, device=self.dev)
+        grad = torch.empty((n, 3), dtype=torch.float32, device=self.dev)
+        call("vx_sdf_grad_fwd", 0, ptr(self.sdf), self.sdf_geom.data_ptr(), ptr(pts),
+             ptr(self.xyz_min), ptr(self.xyz_max), float(self.cfg.voxel_size), n, ptr(sdf),
+             ptr(grad), None, stream_of(pts))
+        normal = grad / (grad.norm(dim=-1, keepdim=True) + 1e-5)
+        return sdf, normal
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    kind: str                    # "render" | "sdf"
+    scene: int
+    rays: tuple = None           # (rays_o, rays_d, viewdirs) on the device, for renders
+    tile: tuple = None           # (res, (i0, j0, k0)) for SDF queries
+    arrival: float = 0.0
+    admitted: float = 0.0
+    done_t: float = 0.0
+    out: object = None
+    next_ray: int = 0
+    pending: int = 0             # work items in flight
+    lock: threading.Lock = dataclasses.field(default_factory=threading.Lock)
+
+    @property
+    def n_rays(self):
+        return 0 if self.rays is None else self.rays[0].shape[0]
+
+    def out_bytes(self):
+        if self.kind == "render":
+            return self.n_rays * 12
+        return 64 ** 3 * 16
+
+
+class Server:
+    """Continuous-batching request loop over resident scenes."""
+
+    def __init__(self, scenes, n_workers=3, batch_rays=131072, hbm_budget=8 << 30, bg=0.0):
+        self.scenes = scenes
+        self.batch_rays = batch_rays
+        self.budget = hbm_budget
+        self.bg = bg
+        self.dev = scenes[0].dev
+        self.incoming = queue.Queue()
+        self.waiting = []            # submitted, not yet admitted (arrival order)
+        self.active = []             # admitted, work remaining
+        self.used = 0                # bytes of admitted outputs
+        self.cv = threading.Condition()
+        self.done = []
+        self.stop = False
+        cap = max(s.cap for s in scenes)
+        self.workspaces = [Workspace(self.dev, cap) for _ in range(n_workers)]
+        for ws in self.workspaces:
+            ws.ensure(batch_rays)
+        self.threads = [threading.Thread(target=self._worker, args=(ws,), daemon=True)
+                        for ws in self.workspaces]
+        for t in self.threads:
+            t.start()
+
+    def submit(self, req):
+        req.arrival = time.perf_counter()
+        with self.cv:
+            self.waiting.append(req)
+            self.cv.notify_all()
+
+    def _admit(self):
+        # oldest first; a request that does not fit blocks later ones (no starvation)
+        while self.waiting:
+            r = self.waiting[0]
+            need = r.out_bytes()
+            if self.used + need > self.budget and self.used > 0:
+                break
+            self.waiting.pop(0)
+            r.admitted = time.perf_counter()
+            self.used += need
+            if r.kind == "render":
+                r.out = torch.empty((r.n_rays, 3), dtype=torch.float32, device=self.dev)
+            self.active.append(r)
+
+    def _next_item(self):
+        """A work item: an SDF query, or up to batch_rays rays of one scene from the
+        oldest admitted requests (continuous batching)."""
+        self._admit()
+        for r in self.active:
+            if r.kind == "sdf" and r.pending == 0 and r.next_ray == 0:
+                r.next_ray = 1
+                r.pending += 1
+                return ("sdf", r.scene, [(r, 0, 0)])
+        segs, scene, room = [], None, self.batch_rays
+        for r in self.active:
+            if r.kind != "render" or r.next_ray >= r.n_rays:
+                continue
+            if scene is None:
+                scene = r.scene
+            if r.scene != scene:
+                continue
+            take = min(room, r.n_rays - r.next_ray)
+            segs.append((r, r.next_ray, take))
+            r.next_ray += take
+            r.pending += 1
+            room -= take
+            if room == 0:
+                break
+        if segs:
+            return ("render", scene, segs)
+        return None
+
+    def _finish(self, r):
+        r.done_t = time.perf_counter()
+        self.used -= r.out_bytes()
+        self.active.remove(r)
+        self.done.append(r)
+
+    def _worker(self, ws):
+        while True:
+            with self.cv:
+                item = None
+                while not self.stop:
+                    item = self._next_item()
+                    if item is not None:
+                        break
+                    self.cv.wait(0.002)
+                if item is None:
+                    return
+            kind, scene, segs = item
+            sc = self.scenes[scene]
+            with torch.cuda.stream(ws.stream):
+                if kind == "sdf":
+                    r = segs[0][0]
+                    res, origin = r.tile
+                    r.out = sc.sdf_query(res, origin)
+                else:
+                    if len(segs) == 1:
+                        r, a, k = segs[0]
+                        o, d, v = (x[a:a + k] for x in r.rays)
+                    else:
+                        o = torch.cat([r.rays[0][a:a + k] for r, a, k in segs])
+                        d = torch.cat([r.rays[1][a:a + k] for r, a, k in segs])
+                        v = torch.cat([r.rays[2][a:a + k] for r, a, k in segs])
+                    rgb = sc.render(o, d, v, ws, self.bg)
+                    pos = 0
+                    for r, a, k in segs:
+                        r.out[a:a + k].copy_(rgb[pos:pos + k])
+                        pos += k
+                ws.stream.synchronize()
+            with self.cv:
+                for r, _, _ in segs:
+                    r.pending -= 1
+                    finished = r.pending == 0 and (r.kind == "sdf" or r.next_ray >= r.n_rays)
+                    if finished and r in self.active:
+                        self._finish(r)
+                self.cv.notify_all()
+
+    def drain(self, n_expected, timeout=600.0):
+        t0 = time.perf_counter()
+        with self.cv:
+            while len(self.done) < n_expected:
+                if time.perf_counter() - t0 > timeout:
+                    raise TimeoutError(f"{len(self.done)}/{n_expected} requests done")
+                self.cv.wait(0.01)
+
+    def close(self):
+        with self.cv:
+            self.stop = True
+            self.cv.notify_all()
+        for t in self.threads:
+            t.join()
+
+
+def sequential(scenes, requests, chunk=8192, bg=0.0, arrivals=None):
+    """The reference's serving discipline: one request at a time in arrival order, each
+    rendered alone in `chunk`-ray pieces (run.py:123-126).  Each request starts at
+    max(its arrival, the previous completion); returns the requests with times set."""
+    ws = Workspace(scenes[0].dev, max(s.cap for s in scenes))
+    ws.ensure(chunk)
+    t0 = time.perf_counter()
+    for i, r in enumerate(requests):
+        if arrivals is not None:
+            while time.perf_counter() - t0 < arrivals[i]:
+                time.sleep(0.0002)
+            r.arrival = t0 + arrivals[i]
+        sc = scenes[r.scene]
+        with torch.cuda.stream(ws.stream):
+            if r.kind == "sdf":
+                res, origin = r.tile
+                r.out = sc.sdf_query(res, origin)
+            else:
+                r.out = torch.empty((r.n_rays, 3), dtype=torch.float32, device=sc.dev)
+                for a in range(0, r.n_rays, chunk):
+                    o, d, v = (x[a:a + chunk] for x in r.rays)
+                    r.out[a:a + o.shape[0]].copy_(sc.render(o, d, v, ws, bg))
+            ws.stream.synchronize()
+        r.done_t = time.perf_counter()
+    return requests
+
+
+def camera_rays(n, seed, device, H=1200, W=1600, focal=1400.0, radius=3.0):
+    """n rays of a random pinhole view on the radius-3 sphere looking at the origin: a
+    random contiguous run of n pixels of an H x W image (row-major)."""
+    g = torch.Generator().manual_seed(seed)
+    u = torch.randn(3, generator=g, dtype=torch.float64)
+    u = u / u.norm()
+    c = radius * u
+    fwd = -u
+    up = torch.tensor([0.0, 0.0, 1.0], dtype=torch.float64)
+    if abs(float(fwd @ up)) > 0.99:
+        up = torch.tensor([0.0, 1.0, 0.0], dtype=torch.float64)
+    right = torch.linalg.cross(fwd, up)
+    right = right / right.norm()
+    up2 = torch.linalg.cross(right, fwd)
+    start = int(torch.randint(0, max(1, H * W - n + 1), (1,), generator=g))
+    idx = torch.arange(start, start + n, device=device)
+    py = (idx // W).double()
+    px = (idx % W).double()
+    R = torch.stack([right, up2, fwd], 1).to(device)
+    dcam = torch.stack([(px - W / 2) / focal, -(py - H / 2) / focal, torch.ones_like(px)], -1)
+    d = dcam @ R.T
+    d = d / d.norm(dim=-1, keepdim=True)
+    o = c.to(device).expand_as(d)
+    return o.float().contiguous(), d.float().contiguous(), d.float().contiguous()

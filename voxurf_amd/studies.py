@@ -1,0 +1,263 @@
+"""Measurements beside the headline fine-step line (bench.py --mode ...).
+
+coarse     the plain coarse step (96^3 SDF + RGB grid, 8192 rays, s = 64, TV 1e-3, Adam 0.1)
+recompute  the fine step at 65536 rays: the fused path (per-ray recomputation in the
+           backward) against the op-by-op path (every sample's intermediates stored),
+           peak HBM and rays/s
+precision  rgbnet in BF16 (FP32 accumulation) vs FP32, against a float64 evaluation of
+           the same model (float64 HIP ops + a float64 rgbnet), over seeds; then the PSNR
+           of 200 training steps with each
+serve      the continuous-batching server vs sequential per-request rendering on a
+           Poisson request stream over 4 resident 256^3 scenes
+"""
+import dataclasses
+import math
+import time
+
+import torch
+import torch.nn.functional as F
+
+from . import scene
+from .fused import FusedFineTrainer
+from .model import Trainer, VoxurfCoarse, VoxurfFine
+
+
+def _time_steps(step, n_steps, n_warm):
+    for i in range(n_warm):
+        step(i)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(n_steps + 1)]
+    ev[0].record()
+    for i in range(n_steps):
+        step(n_warm + i)
+        ev[i + 1].record()
+    torch.cuda.synchronize()
+    per = sorted(ev[i].elapsed_time(ev[i + 1]) for i in range(n_steps))
+    return sum(per) / len(per), per
+
+
+def _pool(cfg, n, dev, base_seed=100):
+    out = []
+    for i in range(n):
+        o, d, v = scene.rays(cfg.n_rays, seed=base_seed + i)
+        out.append((o.to(dev), d.to(dev), v.to(dev), scene.targets(cfg.n_rays, seed=base_seed + i).to(dev)))
+    return out
+
+
+def coarse(args):
+    cfg = scene.COARSE
+    dev = torch.device("cuda:0")
+    pool = _pool(cfg, 4, dev)
+    model = VoxurfCoarse(cfg, scene.sdf_grid(cfg.R), scene.feature_grid(cfg.R, 3)).to(dev)
+    tr = Trainer(model, cfg)
+    counts = {}
+
+    def step(i):
+        out = tr.step(*pool[i % 4])
+        counts["m2"] = out["n_samples"]
+
+    ms, per = _time_steps(step, args.steps, args.warmup)
+    return {"metric": "coarse train-step rays/s at 96^3", "value": cfg.n_rays / (ms * 1e-3),
+            "unit": "rays/s", "ms_per_step": ms, "p50_ms": per[len(per) // 2], "steps": args.steps,
+            "config": {"workload": "plain coarse step: 96^3 SDF + 96^3x3 RGB grid, 8192 rays, s=64, "
+                                   "TV 1e-3, Adam lr 0.1", "path": "ops"},
+            "samples": counts, "dtype": "f32", "data": "synthetic"}
+
+
+def recompute(args):
+    cfg = dataclasses.replace(scene.FINE, n_rays=65536)
+    dev = torch.device("cuda:0")
+    pool = _pool(cfg, 2, dev)
+    res = {"metric": "fine step at 65536 rays: per-ray recomputation vs store-all",
+           "config": {"workload": "plain fine step at 65536 rays (256^3 SDF + 256^3x12 features)"}}
+    for path in ("fused", "ops"):
+        sdf = scene.sdf_grid(cfg.R).to(dev)
+        k0 = scene.feature_grid(cfg.R, 12).to(dev)
+        mlp = scene.mlp_params(cfg.mlp_in, 192, 4)
+        if path == "fused":
+            t = FusedFineTrainer(cfg, sdf, k0, mlp, device=dev)
+        else:
+            t = Trainer(VoxurfFine(cfg, sdf, k0, mlp).to(dev), cfg)
+        del sdf, k0
+        torch.cuda.synchronize()
+        base = torch.cuda.memory_allocated()
+        t.step(*pool[0])  # allocates optimiser state and workspaces
+        torch.cuda.synchronize()
+        torch.cuda.reset_peak_memory_stats()
+        base = torch.cuda.memory_allocated()
+        ms, per = _time_steps(lambda i: t.step(*pool[i % 2]), max(3, args.steps // 4), 1)
+        peak = torch.cuda.max_memory_allocated() - base
+        res[path] = {"rays_per_s": cfg.n_rays / (ms * 1e-3), "ms_per_step": ms,
+                     "peak_step_bytes": int(peak),
+                     "resident_bytes": int(base)}
+        del t
+        torch.cuda.empty_cache()
+    res["speedup"] = res["fused"]["rays_per_s"] / res["ops"]["rays_per_s"]
+    res["peak_ratio"] = res["ops"]["peak_step_bytes"] / max(1, res["fused"]["peak_step_bytes"])
+    return res
+
+
+def _fp64_model(cfg, sdf, k0, mlp, dev):
+    m = VoxurfFine(cfg, sdf, k0, mlp).double().to(dev)
+    m.rgbnet.compute_dtype = torch.float64
+    return m
+
+
+def precision(args):
+    """C4: coarse-resolution scene (96^3) with the 12-channel features and the fine
+    rgbnet, 65536 rays."""
+    cfg = dataclasses.replace(scene.FINE, R=96, n_rays=65536)
+    dev = torch.device("cuda:0")
+    rows = []
+    for seed in range(args.seeds):
+        sdf = scene.sdf_grid(cfg.R)
+        k0 = scene.feature_grid(cfg.R, 12, seed=seed)
+        mlp = scene.mlp_params(cfg.mlp_in, 192, 4, seed=seed)
+        o, d, v = (x.to(dev) for x in scene.rays(cfg.n_rays, seed=seed))
+        tg = scene.targets(cfg.n_rays, seed=seed).to(dev)
+        ref = _fp64_model(cfg, sdf, k0, mlp, dev)
+        out = ref(o.double(), d.double(), v.double())
+        loss = F.mse_loss(out["rgb_marched"], tg.double())
+        loss.backward()
+        g_ref = {"sdf": ref.sdf.grid.grad.float(), "k0": ref.k0.grid.grad.float()}
+        rgb_ref = out["rgb_marched"].detach()
+        row = {"seed": seed, "m2_fp64": out["n_samples"]}
+        for name, dt in (("fp32", torch.float32), ("bf16", torch.bfloat16)):
+            t = FusedFineTrainer(cfg, sdf, k0, mlp, device=dev, mlp_dtype=dt)
+            r = t.step(o, d, v, tg, profile=True)
+            gs = t.grads_snapshot
+            e_rgb = (r["rgb_marched"].double() - rgb_ref).abs()
+            row[name] = {
+                "m2": t.m,
+                "rgb_max_abs_err": float(e_rgb.max()),
+                "rgb_mean_abs_err": float(e_rgb.mean()),
+                "sdf_grad_rel_l2": float((gs["sdf"] - g_ref["sdf"]).norm() / g_ref["sdf"].norm()),
+                "k0_grad_rel_l2": float((gs["k0"].contiguous() - g_ref["k0"]).norm() / g_ref["k0"].norm()),
+            }
+            del t
+        rows.append(row)
+        del ref
+        torch.cuda.empty_cache()
+    summary = {}
+    for name in ("fp32", "bf16"):
+        for k in ("rgb_max_abs_err", "rgb_mean_abs_err", "sdf_grad_rel_l2", "k0_grad_rel_l2"):
+            vals = [r[name][k] for r in rows]
+            summary[f"{name}_{k}_max_over_seeds"] = max(vals)
+    # PSNR over training: both paths from one init, the same ray/target stream; targets
+    # are renders of a second scene (a different feature grid and rgbnet)
+    psnr = {}
+    n_pool = 8
+    sdf = scene.sdf_grid(cfg.R)
+    k0 = scene.feature_grid(cfg.R, 12, seed=0)
+    mlp = scene.mlp_params(cfg.mlp_in, 192, 4, seed=0)
+    truth = FusedFineTrainer(cfg, sdf, scene.feature_grid(cfg.R, 12, seed=99),
+                             scene.mlp_params(cfg.mlp_in, 192, 4, seed=99), device=dev)
+    batches = []
+    for i in range(n_pool + 1):
+        o, d, v = (x.to(dev) for x in scene.rays(cfg.n_rays, seed=1000 + i))
+        with torch.no_grad():
+            truth.forward(o, d, v)
+            tg = _render_with(truth, o, d, v)
+        batches.append((o, d, v, tg))
+    held = batches[-1]
+    for name, dt in (("fp32", torch.float32), ("bf16", torch.bfloat16)):
+        t = FusedFineTrainer(cfg, sdf, k0, mlp, device=dev, mlp_dtype=dt)
+        for i in range(args.psnr_steps):
+            t.step(*batches[i % n_pool])
+        o, d, v, tg = held
+        t.forward(o, d, v)
+        pred = _render_with(t, o, d, v)
+        mse = float(F.mse_loss(pred, tg))
+        psnr[name] = -10.0 * math.log10(mse)
+        del t
+    return {"metric": "rgbnet BF16 vs FP32 vs FP64 (96^3 scene, 65536 rays)", "seeds": rows,
+            "summary": summary, "psnr_after_steps": psnr, "psnr_steps": args.psnr_steps,
+            "psnr_drift_db": psnr["bf16"] - psnr["fp32"],
+            "config": {"workload": "96^3 SDF + 96^3x12 features, rgbnet 42-192x3-3 (FP32 vs BF16 "
+                                   "MFMA, FP32 accumulation), 65536 rays"}}
+
+
+@torch.no_grad()
+def _render_with(t, o, d, v):
+    """rgb of the rays whose survivors t.forward() just built (t.X etc.)."""
+    from ._lib import call, ptr, stream_of
+    m = t.m
+    n = o.shape[0]
+    logits = t.rgbnet(t.X[:m, :t.cfg.mlp_in]).contiguous() if m > 0 else \
+        torch.zeros((1, 3), device=o.device)
+    out = torch.empty((n, 3), device=o.device)
+    call("vx_fused_fine_composite", n, ptr(t.cnt), ptr(t.off), ptr(logits), 3, ptr(t.P4),
+         ptr(t.last), None, float(t.cfg.bg), ptr(out), None, None, None, None, None, stream_of(o))
+    return out
+
+
+def serve(args):
+    """C5: 4 resident 256^3 scenes; Poisson arrivals; 80% renders of log-uniform
+    1e3-1.92e6 rays (runs of pixels of random 1600x1200 views on the radius-3 sphere),
+    20% SDF/normal queries of a 64^3 tile of the 512^3 lattice."""
+    import random
+    from .serve import Request, ResidentScene, Server, camera_rays, sequential
+    dev = torch.device("cuda:0")
+    cfg = scene.FINE
+    scenes = [ResidentScene(cfg, scene.sdf_grid(cfg.R), scene.feature_grid(cfg.R, 12, seed=s),
+                            scene.mlp_params(cfg.mlp_in, 192, 4, seed=s), device=dev) for s in range(4)]
+    rng = random.Random(0)
+    n_req = args.requests
+    arrivals, t = [], 0.0
+    specs = []
+    for i in range(n_req):
+        t += rng.expovariate(args.rate)
+        arrivals.append(t)
+        if rng.random() < 0.8:
+            n = int(round(math.exp(rng.uniform(math.log(1e3), math.log(1.92e6)))))
+            specs.append(("render", rng.randrange(4), n, i))
+        else:
+            specs.append(("sdf", rng.randrange(4), (512, (64 * rng.randrange(8), 64 * rng.randrange(8),
+                                                           64 * rng.randrange(8))), i))
+
+    def build():
+        reqs = []
+        for kind, s, x, i in specs:
+            if kind == "render":
+                reqs.append(Request(rid=i, kind="render", scene=s, rays=camera_rays(x, seed=i, device=dev)))
+            else:
+                reqs.append(Request(rid=i, kind="sdf", scene=s, tile=x))
+        torch.cuda.synchronize()
+        return reqs
+
+    def stats(reqs, t_first, t_last):
+        lat = sorted((r.done_t - r.arrival) * 1e3 for r in reqs)
+        rays = sum(r.n_rays for r in reqs)
+        span = t_last - t_first
+        return {"p50_ms": lat[len(lat) // 2], "p99_ms": lat[min(len(lat) - 1, int(0.99 * len(lat)))],
+                "max_ms": lat[-1], "rays_per_s": rays / span, "makespan_s": span, "requests": len(reqs)}
+
+    # warm-up (kernels, allocator)
+    warm = [Request(rid=-1, kind="render", scene=0, rays=camera_rays(100000, seed=12345, device=dev))]
+    sequential(scenes, warm)
+    # continuous batching
+    reqs = build()
+    srv = Server(scenes, n_workers=3, batch_rays=131072, hbm_budget=16 << 30)
+    t0 = time.perf_counter()
+    for r, a in zip(reqs, arrivals):
+        while time.perf_counter() - t0 < a:
+            time.sleep(0.0002)
+        srv.submit(r)
+    srv.drain(len(reqs), timeout=1200)
+    srv.close()
+    batched = stats(reqs, t0, max(r.done_t for r in reqs))
+    out_b = {r.rid: r.out for r in reqs}
+    # sequential per-request, same arrival trace
+    reqs2 = build()
+    t0 = time.perf_counter()
+    sequential(scenes, reqs2, chunk=8192, arrivals=arrivals)
+    seq = stats(reqs2, t0, max(r.done_t for r in reqs2))
+    same = all((torch.equal(out_b[r.rid], r.out) if r.kind == "render" else
+                torch.equal(out_b[r.rid][0], r.out[0])) for r in reqs2)
+    total_rays = sum(r.n_rays for r in reqs)
+    return {"metric": "render serving p50/p99 latency (ms) and rays/s", "continuous_batching": batched,
+            "sequential_8192_chunks": seq, "identical_results": same, "total_rays": total_rays,
+            "config": {"workload": f"4 resident 256^3 scenes (SDF + 12-ch features + rgbnet), "
+                                   f"{n_req} requests, Poisson {args.rate}/s, 80% renders of "
+                                   f"log-uniform 1e3-1.92e6 rays, 20% 64^3 SDF/normal tiles of "
+                                   f"the 512^3 lattice", "workers": 3, "batch_rays": 131072}}
