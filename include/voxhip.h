@@ -186,10 +186,13 @@ int vx_adam_upd(int dtype, void* param, const void* grad, void* exp_avg, void* e
 
 /* lib/utils.py:154-199 adam() for one parameter (the optimiser the training step
  * runs): step is the 1-based step count after increment; per_lr may be NULL;
- * clear_grad != 0 zeroes grad after use. */
+ * clear_grad != 0 zeroes grad after use.  pages (may be NULL; float32 with clear_grad
+ * only): one flag per 1024-element page of grad, nonzero where anything was added to
+ * that page since the last step; pages flagged zero are all-zero gradient and are
+ * neither read nor cleared; the flags are reset to zero. */
 int vx_adam_step(int dtype, void* param, void* grad, void* exp_avg, void* exp_avg_sq,
                  const void* per_lr, int64_t n, int step, double lr, double beta1, double beta2,
-                 double eps, int clear_grad, void* stream);
+                 double eps, int clear_grad, uint32_t* pages, void* stream);
 
 /* ---- per-sample ops ------------------------------------------------------------ */
 
@@ -243,11 +246,12 @@ int vx_fused_fine_composite(int64_t n_rays, const int64_t* cnt, const int64_t* o
                             const float* target, float bg, float* rgb, float* dlogit, float* gw,
                             float* glast, float* S, float* loss_r, void* stream);
 /* Adjoint of the feature rows: k0_grad += trilinear scatter of dX[:, :C]; dgf [M,3] =
- * d loss / d gradient through the normalised normal dX[:, C:C+3]. */
+ * d loss / d gradient through the normalised normal dX[:, C:C+3].  k0_pages (may be
+ * NULL): the page flags of k0_grad for vx_adam_step, set for every page written. */
 int vx_fused_fine_feat_bwd(const int64_t* k0_geom, const int64_t* sdf_geom, const float* box,
                            const float* params, int64_t m, const float* dX, int dxs,
                            const void* P4, const void* G4, float* k0_grad, float* dgf,
-                           void* stream);
+                           uint32_t* k0_pages, void* stream);
 /* Per-ray recomputing backward of the SDF side: sdf_grad += adjoint of (7-point SDF
  * lookup -> NeuS alpha -> transmittance walk) for every sample of the walk range. */
 int vx_fused_fine_bwd(const float* sdf, const int64_t* sdf_geom, const float* box,

@@ -158,7 +158,7 @@ def test_adam_kernels(dev, dtype):
             from voxurf_amd._lib import call, dtype_code, ptr, stream_of
             call("vx_adam_step", dtype_code(Pd), ptr(Pd), ptr(Gd), ptr(Md), ptr(Vd),
                  ptr(None if per is None else per.to(dev)), n, step, 0.1, 0.9, 0.99, 1e-8, 1,
-                 stream_of(Pd))
+                 None, stream_of(Pd))
             close(Pd, P, tol(dtype))
             close(Md, M, tol(dtype))
             close(Vd, V, tol(dtype))

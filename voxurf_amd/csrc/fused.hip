@@ -266,9 +266,14 @@ __global__ void k_fused_composite(int64_t n_rays, const int64_t* __restrict__ cn
 
 // Feature-side adjoint per survivor: dX[:, 0:C] -> k0 grad (trilinear scatter);
 // dX[:, C:C+3] (normal) -> d grad through n = g / (|g| + 1e-5); result stored in dgf.
+__device__ __forceinline__ void mark_page(uint32_t* pages, int64_t addr) {
+  pages[addr >> 10] = 1u;
+}
+
 __global__ void k_fused_feat_bwd(FusedGeom G, int64_t m, const float* __restrict__ dX, int dxs,
                                  const float4* __restrict__ P4, const float4* __restrict__ G4,
-                                 float* __restrict__ k0_grad, float* __restrict__ dgf) {
+                                 float* __restrict__ k0_grad, float* __restrict__ dgf,
+                                 uint32_t* __restrict__ pages) {
   const int C = G.gk.C;
   for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < m;
        j += (int64_t)gridDim.x * blockDim.x) {
@@ -278,6 +283,23 @@ __global__ void k_fused_feat_bwd(FusedGeom G, int64_t m, const float* __restrict
            world_to_index(a.y, G.mn[1], G.mx[1] - G.mn[1], G.gk.Y),
            world_to_index(a.z, G.mn[2], G.mx[2] - G.mn[2], G.gk.Z));
     for (int ch = 0; ch < C; ++ch) tri_scatter(k0_grad, G.gk, t, ch, dX[j * dxs + ch]);
+    if (pages) {
+      // flag every page the scatter may have written: per in-grid corner, its channels
+      // (one run of C floats when channels are innermost, C strided words otherwise)
+      for (int bx = 0; bx < 2; ++bx)
+        for (int by = 0; by < 2; ++by)
+          for (int bz = 0; bz < 2; ++bz) {
+            const int x = t.x0 + bx, y = t.y0 + by, z = t.z0 + bz;
+            if (!in_grid<float>(x, y, z, G.gk)) continue;
+            const int64_t a = x * G.gk.sx + y * G.gk.sy + z * G.gk.sz;
+            if (G.gk.sc == 1) {
+              mark_page(pages, a);
+              mark_page(pages, a + C - 1);
+            } else {
+              for (int ch = 0; ch < C; ++ch) mark_page(pages, a + ch * G.gk.sc);
+            }
+          }
+    }
     const float4 b = G4[j];
     const float gvec[3] = {b.x, b.y, b.z};
     const float dn[3] = {dX[j * dxs + C], dX[j * dxs + C + 1], dX[j * dxs + C + 2]};
@@ -446,12 +468,12 @@ int vx_fused_fine_composite(int64_t n_rays, const int64_t* cnt, const int64_t* o
 int vx_fused_fine_feat_bwd(const int64_t* k0_geom, const int64_t* sdf_geom, const float* box,
                            const float* params, int64_t m, const float* dX, int dxs,
                            const void* P4, const void* G4, float* k0_grad, float* dgf,
-                           void* stream) {
+                           uint32_t* k0_pages, void* stream) {
   FusedGeom G;
   if (int st = fused_geom(G, nullptr, sdf_geom, nullptr, k0_geom, box, params)) return st;
   if (m == 0) return VX_OK;
   k_fused_feat_bwd<<<grid_for(m, 256), 256, 0, (hipStream_t)stream>>>(
-      G, m, dX, dxs, (const float4*)P4, (const float4*)G4, k0_grad, dgf);
+      G, m, dX, dxs, (const float4*)P4, (const float4*)G4, k0_grad, dgf, k0_pages);
   return launch_status("fused_fine_feat_bwd");
 }
 

@@ -44,10 +44,26 @@ __device__ __forceinline__ void adam_elem(float& p, float g, float& m, float& v,
   p = p + c.neg_step * ((m * plr) / denom);
 }
 
+// Streaming pass over 16-byte vectors.  `pages` (optional) holds one flag per
+// kPage-element page of the gradient: a zero flag means the whole page of the gradient
+// is zero (nothing was scattered into it this step), so it is neither read nor cleared.
+constexpr int kPageShift = 10;  // 1024 elements
+
+typedef float nt_f4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 ldnt(const float4* p) {
+  const nt_f4 v = __builtin_nontemporal_load(reinterpret_cast<const nt_f4*>(p));
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void stnt(float4* p, float4 v) {
+  const nt_f4 w = {v.x, v.y, v.z, v.w};
+  __builtin_nontemporal_store(w, reinterpret_cast<nt_f4*>(p));
+}
+
 __global__ void __launch_bounds__(256) k_adam_step_f32(float* __restrict__ p, float* __restrict__ g,
                                                        float* __restrict__ m, float* __restrict__ v,
                                                        const float* __restrict__ perlr, int64_t n,
-                                                       AdamCoef c, int clear, int vec) {
+                                                       AdamCoef c, int clear, int vec,
+                                                       const uint32_t* __restrict__ pages) {
   const int64_t n4 = vec ? (n >> 2) : 0;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   float4* p4 = reinterpret_cast<float4*>(p);
@@ -56,25 +72,34 @@ __global__ void __launch_bounds__(256) k_adam_step_f32(float* __restrict__ p, fl
   float4* v4 = reinterpret_cast<float4*>(v);
   const float4* l4 = reinterpret_cast<const float4*>(perlr);
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n4; i += stride) {
-    float4 pp = p4[i], gg = g4[i], mm = m4[i], vv = v4[i];
+    const bool live = !pages || pages[(i << 2) >> kPageShift] != 0;
+    float4 pp = ldnt(p4 + i), mm = ldnt(m4 + i), vv = ldnt(v4 + i);
+    float4 gg = live ? ldnt(g4 + i) : make_float4(0.f, 0.f, 0.f, 0.f);
     const float4 ll = perlr ? l4[i] : make_float4(1.f, 1.f, 1.f, 1.f);
     adam_elem(pp.x, gg.x, mm.x, vv.x, ll.x, c);
     adam_elem(pp.y, gg.y, mm.y, vv.y, ll.y, c);
     adam_elem(pp.z, gg.z, mm.z, vv.z, ll.z, c);
     adam_elem(pp.w, gg.w, mm.w, vv.w, ll.w, c);
-    p4[i] = pp;
-    m4[i] = mm;
-    v4[i] = vv;
-    if (clear) g4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    stnt(p4 + i, pp);
+    stnt(m4 + i, mm);
+    stnt(v4 + i, vv);
+    if (clear && live) stnt(g4 + i, make_float4(0.f, 0.f, 0.f, 0.f));
   }
   for (int64_t i = (n4 << 2) + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride) {
+    const bool live = !pages || pages[i >> kPageShift] != 0;
     float pp = p[i], mm = m[i], vv = v[i];
-    adam_elem(pp, g[i], mm, vv, perlr ? perlr[i] : 1.f, c);
+    adam_elem(pp, live ? g[i] : 0.f, mm, vv, perlr ? perlr[i] : 1.f, c);
     p[i] = pp;
     m[i] = mm;
     v[i] = vv;
-    if (clear) g[i] = 0.f;
+    if (clear && live) g[i] = 0.f;
   }
+}
+
+__global__ void k_zero_u32(uint32_t* __restrict__ p, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0u;
 }
 
 __global__ void k_adam_step_f64(double* __restrict__ p, double* __restrict__ g,
@@ -127,8 +152,10 @@ int vx_adam_upd(int dtype, void* param, const void* grad, void* exp_avg, void* e
 
 int vx_adam_step(int dtype, void* param, void* grad, void* exp_avg, void* exp_avg_sq,
                  const void* per_lr, int64_t n, int step, double lr, double beta1, double beta2,
-                 double eps, int clear_grad, void* stream) {
+                 double eps, int clear_grad, uint32_t* pages, void* stream) {
   VX_CHECK_ARG(n >= 0, "adam_step: n < 0");
+  VX_CHECK_ARG(!pages || (dtype == VX_F32 && clear_grad),
+               "adam_step: page flags need float32 and clear_grad");
   VX_CHECK_ARG(step >= 1, "adam_step: step must be >= 1");
   if (n == 0) return VX_OK;
   const double bc1 = 1.0 - pow(beta1, (double)step);
@@ -151,7 +178,11 @@ int vx_adam_step(int dtype, void* param, void* grad, void* exp_avg, void* exp_av
     const int64_t work = vec ? n / 4 + (n & 3) : n;
     k_adam_step_f32<<<grid_for(work, 256, 8192), 256, 0, st>>>(
         (float*)param, (float*)grad, (float*)exp_avg, (float*)exp_avg_sq, (const float*)per_lr,
-        n, c, clear_grad, vec);
+        n, c, clear_grad, vec, pages);
+    if (pages) {
+      const int64_t np = (n + (1 << kPageShift) - 1) >> kPageShift;
+      k_zero_u32<<<grid_for(np, 256), 256, 0, st>>>(pages, np);
+    }
   } else if (dtype == VX_F64) {
     k_adam_step_f64<<<grid_for(n, 256), 256, 0, st>>>(
         (double*)param, (double*)grad, (double*)exp_avg, (double*)exp_avg_sq,

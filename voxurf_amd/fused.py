@@ -40,6 +40,10 @@ class FusedFineTrainer:
         self.k0 = k0.to(self.dev, torch.float32).contiguous(memory_format=torch.channels_last_3d)
         self.sdf_grad = torch.zeros_like(self.sdf)
         self.k0_grad = torch.zeros_like(self.k0)
+        # one flag per 1024-element page of k0_grad: set by the feature scatter, consumed
+        # and reset by Adam, which skips the gradient traffic of untouched pages
+        self.k0_pages = torch.zeros((self.k0.numel() + 1023) // 1024, dtype=torch.int32,
+                                    device=self.dev)
         self.adam = {
             "sdf": [torch.zeros_like(self.sdf), torch.zeros_like(self.sdf), 0, cfg.lr_sdf],
             "k0": [torch.zeros_like(self.k0), torch.zeros_like(self.k0), 0, cfg.lr_k0],
@@ -80,7 +84,7 @@ class FusedFineTrainer:
         self.loss = torch.empty(1, dtype=torch.float32, device=d)
         self._n = n
 
-    def _adam(self, name, p, g, profile=False):
+    def _adam(self, name, p, g, profile=False, pages=None):
         m, v, step, lr = self.adam[name]
         step += 1
         self.adam[name][2] = step
@@ -89,7 +93,7 @@ class FusedFineTrainer:
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record()
         call("vx_adam_step", 0, ptr(p), ptr(g), ptr(m), ptr(v), None, p.numel(), step, float(lr),
-             0.9, 0.99, 1e-8, 1, st)
+             0.9, 0.99, 1e-8, 1, ptr(pages), st)
         if profile:
             e1.record()
             return (e0, e1)
@@ -144,7 +148,7 @@ class FusedFineTrainer:
         dgf = torch.empty((max(m, 1), 3), dtype=torch.float32, device=self.dev)
         call("vx_fused_fine_feat_bwd", self.k0_geom.data_ptr(), self.sdf_geom.data_ptr(),
              self.box.data_ptr(), self.params.data_ptr(), m, ptr(dX), dX.stride(0), ptr(self.P4),
-             ptr(self.G4), ptr(self.k0_grad), ptr(dgf), st)
+             ptr(self.G4), ptr(self.k0_grad), ptr(dgf), ptr(self.k0_pages), st)
         call("vx_fused_fine_bwd", ptr(self.sdf), self.sdf_geom.data_ptr(), self.box.data_ptr(),
              self.params.data_ptr(), ptr(rays_o), ptr(rays_d), ptr(viewdirs), n, ptr(self.cnt),
              ptr(self.off), ptr(self.last), ptr(gw), ptr(self.glast), ptr(self.S), ptr(dgf),
@@ -153,7 +157,7 @@ class FusedFineTrainer:
             self.grads_snapshot = dict(sdf=self.sdf_grad.clone(), k0=self.k0_grad.clone(),
                                        mlp=[p.grad.clone() for p in self.rgbnet.parameters()])
         self._adam("sdf", self.sdf, self.sdf_grad)
-        ev = self._adam("k0", self.k0, self.k0_grad, profile=profile)
+        ev = self._adam("k0", self.k0, self.k0_grad, profile=profile, pages=self.k0_pages)
         self.mlp_opt.step()
         call("vx_sum_f32", ptr(self.loss_r), n, ptr(self.loss), st)
         if profile:

@@ -75,7 +75,7 @@ class Adam(torch.optim.Optimizer):
                 call("vx_adam_step", dtype_code(p), ptr(p), ptr(g), ptr(state["exp_avg"]),
                      ptr(state["exp_avg_sq"]), ptr(per_lr), p.numel(), state["step"],
                      float(group["lr"]), float(beta1), float(beta2), float(group["eps"]),
-                     int(self.clear_grad), stream_of(p))
+                     int(self.clear_grad), None, stream_of(p))
                 if ev is not None:
                     e1.record()
                     ev[id(p)] = (e0, e1)
