@@ -34,7 +34,8 @@ def parse():
     ap.add_argument("--rays", type=int, default=8192)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--mode", choices=["train", "coarse", "recompute", "precision", "serve"],
+    ap.add_argument("--mode", choices=["train", "coarse", "recompute", "precision", "serve",
+                                       "kernels"],
                     default="train",
                     help="train: the plain fine step (the headline line); coarse: the plain "
                          "coarse step; recompute: 65536-ray fine step, per-ray recomputation "

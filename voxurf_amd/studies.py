@@ -44,6 +44,74 @@ def _pool(cfg, n, dev, base_seed=100):
     return out
 
 
+def _event_ms(fn, reps=10):
+    fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def kernels(args):
+    """Per-kernel rooflines: the Adam sweep over a 256^3 x 12 grid (with and without
+    page flags), a plain 1 GiB device copy (the calibration kernel for HBM counters),
+    and rgbnet forward+backward at the fine step's survivor count (FP32 and BF16)."""
+    from ._lib import call, ptr, stream_of
+    from .mlp import RgbNet
+    dev = torch.device("cuda:0")
+    res = {"metric": "per-kernel rooflines"}
+    n = 256 ** 3 * 12
+    p, g, m, v = (torch.randn(n, device=dev) * 0.1 for _ in range(4))
+    v = v.abs()
+    st = stream_of(p)
+    step = [0]
+
+    def adam(pages=None):
+        step[0] += 1
+        call("vx_adam_step", 0, ptr(p), ptr(g), ptr(m), ptr(v), None, n, step[0], 0.1, 0.9, 0.99,
+             1e-8, 1, ptr(pages), st)
+
+    ms = _event_ms(adam)
+    res["adam_dense"] = {"ms": ms, "bytes": 32 * n, "GBs": 32 * n / ms / 1e6}
+    pages = torch.zeros((n + 1023) // 1024, dtype=torch.int32, device=dev)
+    frac = 0.25
+
+    def adam_paged():
+        pages[: int(len(pages) * frac)].fill_(1)
+        adam(pages)
+
+    ms = _event_ms(adam_paged)
+    moved = 24 * n + 8 * n * frac
+    res["adam_paged_25pct_live"] = {"ms": ms, "bytes_moved": moved, "GBs_moved": moved / ms / 1e6,
+                                    "algorithmic_GBs": 32 * n / ms / 1e6}
+    del p, g, m, v
+    a = torch.empty(1 << 28, device=dev)
+    b = torch.empty_like(a)
+    ms = _event_ms(lambda: b.copy_(a))
+    res["copy_1GiB"] = {"ms": ms, "bytes": 2 * a.numel() * 4, "GBs": 2 * a.numel() * 4 / ms / 1e6}
+    del a, b
+    M = 155000
+    flops = 2 * M * (42 * 192 + 2 * 192 * 192 + 192 * 3) * 3
+    for name, dt, peak in (("fp32", torch.float32, 157.3), ("bf16", torch.bfloat16, 2500.0)):
+        net = RgbNet(42, 192, 4, compute_dtype=dt).to(dev)
+        x = torch.randn(M, 42, device=dev, requires_grad=True)
+        gout = torch.randn(M, 3, device=dev)
+
+        def fb():
+            out = net(x)
+            out.backward(gout)
+
+        ms = _event_ms(fb, reps=5)
+        res[f"mlp_{name}"] = {"rows": M, "ms_fwd_bwd": ms, "gflop": flops / 1e9,
+                              "TFLOPs": flops / ms / 1e9, "peak_TFLOPs": peak,
+                              "frac": flops / ms / 1e9 / peak}
+    return res
+
+
 def coarse(args):
     cfg = scene.COARSE
     dev = torch.device("cuda:0")
