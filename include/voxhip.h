@@ -173,6 +173,14 @@ int vx_total_variation_add_grad_new(int dtype, const void* param, void* grad, co
                                     const int64_t* geom, float wx, float wy, float wz,
                                     int dense_mode, void* stream);
 
+/* lib/voxurf_fine.py:930-942 MaskCache.forward (also lib/dvgo_ori.py:440-473): out [n]
+ * = 1 - exp(-softplus(density(xyz) + act_shift) * voxel_size_ratio) >= thres, with the
+ * density looked up trilinearly (nearest = 0) or at the nearest lattice point. */
+int vx_maskcache_alpha(int dtype, const void* density, const int64_t* geom, const void* xyz,
+                       const void* xyz_min, const void* xyz_max, double act_shift,
+                       double voxel_size_ratio, double thres, int nearest, int64_t n,
+                       uint8_t* out, void* stream);
+
 /* Zero n elements (a fill kernel, safe inside captured graphs). */
 int vx_zero(int dtype, void* p, int64_t n, void* stream);
 

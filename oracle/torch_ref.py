@@ -144,6 +144,17 @@ def mlp(x, params):
     return h
 
 
+def maskcache_alpha(density, xyz, xyz_min, xyz_max, act_shift, voxel_size_ratio, nearest=False):
+    """lib/voxurf_fine.py:930-942 before the threshold: alpha at xyz from the (already
+    max-pooled) density grid."""
+    shape = xyz.shape[:-1]
+    xyz = xyz.reshape(1, 1, 1, -1, 3)
+    ind_norm = ((xyz - xyz_min) / (xyz_max - xyz_min)).flip((-1,)) * 2 - 1
+    d = F.grid_sample(density, ind_norm, align_corners=True, mode="nearest" if nearest else "bilinear")
+    alpha = 1 - torch.exp(-F.softplus(d + act_shift) * voxel_size_ratio)
+    return alpha.reshape(*shape)
+
+
 # ---- total variation and Adam -----------------------------------------------------------
 
 def tv_add_grad(param, grad, wx, wy, wz, dense_mode, mask=None):

@@ -243,6 +243,41 @@ __global__ void k_tv(const T* __restrict__ param, T* __restrict__ grad, const T*
   }
 }
 
+// ---- MaskCache.forward (lib/voxurf_fine.py:930-942, lib/dvgo_ori.py:440-473) --------
+// density at the point (trilinear, or nearest lattice point with round-half-to-even as
+// grid_sample's 'nearest'), alpha = 1 - exp(-softplus(density + act_shift) * ratio)
+// with torch's softplus (threshold 20), out = alpha >= thres.
+template <typename T>
+__device__ __forceinline__ T softplus20(T x) {
+  return x > (T)20 ? x : log1p(exp(x));
+}
+
+template <typename T>
+__global__ void k_maskcache_alpha(const T* __restrict__ dens, GridDesc g, const T* __restrict__ xyz,
+                                  const T* __restrict__ mn, const T* __restrict__ mx, T act_shift,
+                                  T ratio, T thres, int nearest, int64_t n,
+                                  uint8_t* __restrict__ out) {
+#pragma clang fp contract(off)
+  const T ex = mx[0] - mn[0], ey = mx[1] - mn[1], ez = mx[2] - mn[2];
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    const T fx = world_to_index(xyz[3 * p], mn[0], ex, g.X);
+    const T fy = world_to_index(xyz[3 * p + 1], mn[1], ey, g.Y);
+    const T fz = world_to_index(xyz[3 * p + 2], mn[2], ez, g.Z);
+    T d;
+    if (nearest) {
+      const int x = sat_f2i(rint(fx)), y = sat_f2i(rint(fy)), z = sat_f2i(rint(fz));
+      d = in_grid<T>(x, y, z, g) ? dens[x * g.sx + y * g.sy + z * g.sz] : (T)0;
+    } else {
+      Tri<T> t;
+      t.init(fx, fy, fz);
+      d = tri_gather(dens, g, t, 0);
+    }
+    const T alpha = (T)1 - exp(-softplus20(d + act_shift) * ratio);
+    out[p] = alpha >= thres;
+  }
+}
+
 template <typename T>
 __global__ void k_zero(T* p, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
@@ -417,6 +452,22 @@ int vx_total_variation_add_grad_new(int dtype, const void* param, void* grad, co
   GRID_DISPATCH(dtype, k_tv<scalar_t, true><<<grid_for(N, 256), 256, 0, ST(stream)>>>(
                            CP(param), MP(grad), CP(mask), g, wx, wy, wz, dense_mode));
   return launch_status("total_variation_add_grad_new");
+}
+
+int vx_maskcache_alpha(int dtype, const void* density, const int64_t* geom, const void* xyz,
+                       const void* xyz_min, const void* xyz_max, double act_shift,
+                       double voxel_size_ratio, double thres, int nearest, int64_t n,
+                       uint8_t* out, void* stream) {
+  GridDesc g;
+  if (int st = make_desc(geom, g)) return st;
+  VX_CHECK_ARG(g.C == 1, "maskcache: the density grid must have one channel");
+  VX_CHECK_ARG(n >= 0, "maskcache: n < 0");
+  if (n == 0) return VX_OK;
+  GRID_DISPATCH(dtype, k_maskcache_alpha<scalar_t><<<grid_for(n, 256), 256, 0, ST(stream)>>>(
+                           CP(density), g, CP(xyz), CP(xyz_min), CP(xyz_max),
+                           (scalar_t)act_shift, (scalar_t)voxel_size_ratio, (scalar_t)thres,
+                           nearest, n, out));
+  return launch_status("maskcache_alpha");
 }
 
 int vx_zero(int dtype, void* p, int64_t n, void* stream) {

@@ -49,7 +49,13 @@ class _VoxurfBase(nn.Module):
             rays_o.contiguous(), rays_d.contiguous(), self.xyz_min, self.xyz_max, 0.0, 1e9,
             stepdist)
         mask_inbbox = ~mask_outbbox
-        return ray_pts[mask_inbbox], ray_id[mask_inbbox], step_id[mask_inbbox], mask_outbbox, N_steps
+        ray_pts, ray_id, step_id = ray_pts[mask_inbbox], ray_id[mask_inbbox], step_id[mask_inbbox]
+        if getattr(self, "mask_cache", None) is not None:
+            # known free space from the coarse geometry (lib/voxurf_fine.py:631-636)
+            mask = self.mask_cache(ray_pts)
+            ray_pts, ray_id, step_id = ray_pts[mask], ray_id[mask], step_id[mask]
+            mask_outbbox[~mask_outbbox] |= ~mask
+        return ray_pts, ray_id, step_id, mask_outbbox, N_steps
 
     def sdf_and_gradient(self, ray_pts):
         sdf, gradient, _ = SdfGradSample.apply(self.sdf.grid, ray_pts, self.xyz_min, self.xyz_max,
