@@ -27,6 +27,8 @@ enum { VX_F32 = 0, VX_F64 = 1 };
 
 const char* vx_last_error(void);
 int vx_version(void);
+/* Hash of this header the library was built against (bindings refuse a mismatch). */
+const char* vx_abi_hash(void);
 
 /* ---- render_utils: ray sampling ------------------------------------------------ */
 
@@ -231,9 +233,13 @@ int vx_segment_gather(int dtype, const void* gout, const int64_t* index, int64_t
  * sdf_geom/k0_geom as for the grid ops; box[6] = {xyz_min, xyz_max};
  * params[6] = {stepdist, voxel_size, inv_s, fast_color_thres, near, far}.
  * Per-ray survivor slots hold `cap` entries of two float4 each (s4: x, y, z, weight;
- * g4: gradient xyz, step). */
+ * g4: gradient xyz, step).  mc (may be NULL): the MaskCache density grid (device, one
+ * channel, geometry mc_geom) with mc_params[10] = {act_shift, voxel_size_ratio, thres,
+ * nearest, xyz_min[3], xyz_max[3]}; samples it rejects are dropped like out-of-box ones
+ * (lib/voxurf_fine.py:631-636). */
 int vx_fused_fine_fwd(const float* sdf, const int64_t* sdf_geom, const float* box,
-                      const float* params, const float* rays_o, const float* rays_d,
+                      const float* params, const float* mc, const int64_t* mc_geom,
+                      const float* mc_params, const float* rays_o, const float* rays_d,
                       const float* viewdirs, int64_t n_rays, int cap, void* s4, void* g4,
                       int64_t* cnt, float* last, int* stats, int count_all, int* overflow,
                       void* stream);
@@ -263,7 +269,8 @@ int vx_fused_fine_feat_bwd(const int64_t* k0_geom, const int64_t* sdf_geom, cons
 /* Per-ray recomputing backward of the SDF side: sdf_grad += adjoint of (7-point SDF
  * lookup -> NeuS alpha -> transmittance walk) for every sample of the walk range. */
 int vx_fused_fine_bwd(const float* sdf, const int64_t* sdf_geom, const float* box,
-                      const float* params, const float* rays_o, const float* rays_d,
+                      const float* params, const float* mc, const int64_t* mc_geom,
+                      const float* mc_params, const float* rays_o, const float* rays_d,
                       const float* viewdirs, int64_t n_rays, const int64_t* cnt,
                       const int64_t* off_incl, const float* last, const float* gw,
                       const float* glast, const float* S, const float* dgf, float* sdf_grad,

@@ -75,6 +75,14 @@ def lib():
                     f"{LIB_PATH} is missing: build it with `python -m voxurf_amd.build` "
                     "(the HIP library is required; there is no CPU fallback)")
             h = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_GLOBAL)
+            # the library must have been built against this exact header
+            import hashlib
+            want = hashlib.sha256(open(HEADER_PATH, "rb").read()).hexdigest()[:16]
+            h.vx_abi_hash.restype = ctypes.c_char_p
+            got = h.vx_abi_hash().decode()
+            if got != want:
+                raise VoxhipError(f"{LIB_PATH} was built against another include/voxhip.h "
+                                  f"(hash {got}, header {want}); rebuild it")
             for name, (ret, args) in parse_header().items():
                 fn = getattr(h, name)
                 fn.restype = _CTYPE[ret]

@@ -40,8 +40,18 @@ def _hipcc():
     raise RuntimeError("hipcc not found; the HIP library cannot be built")
 
 
+HEADER = os.path.join(os.path.dirname(HERE), "include", "voxhip.h")
+
+
+def header_hash(path=HEADER):
+    """sha256 of the C-ABI header; compiled into the library and checked at load time."""
+    import hashlib
+    with open(path, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()[:16]
+
+
 def _compile(src, obj):
-    cmd = [_hipcc(), *HIP_FLAGS, "-c", src, "-o", obj]
+    cmd = [_hipcc(), *HIP_FLAGS, f'-DVX_ABI_HASH="{header_hash()}"', "-c", src, "-o", obj]
     p = subprocess.run(cmd, capture_output=True, text=True)
     msgs = "\n".join(l for l in (p.stdout + p.stderr).splitlines()
                      if "packed-fp32-ops" not in l)

@@ -24,7 +24,32 @@ struct FusedGeom {
   GridDesc gk;
   float mn[3], mx[3];
   float stepdist, voxel_size, inv_s, thres, near_, far_;
+  // optional MaskCache (coarse occupancy); mc == nullptr disables it
+  const float* mc;
+  GridDesc gm;
+  float mc_shift, mc_ratio, mc_thres, mc_mn[3], mc_mx[3];
+  int mc_nearest;
 };
+
+// MaskCache.forward for one point (lib/voxurf_fine.py:930-942), as vx_maskcache_alpha.
+__device__ __forceinline__ bool maskcache_pass(const FusedGeom& G, const float q[3]) {
+#pragma clang fp contract(off)
+  const float fx = world_to_index(q[0], G.mc_mn[0], G.mc_mx[0] - G.mc_mn[0], G.gm.X);
+  const float fy = world_to_index(q[1], G.mc_mn[1], G.mc_mx[1] - G.mc_mn[1], G.gm.Y);
+  const float fz = world_to_index(q[2], G.mc_mn[2], G.mc_mx[2] - G.mc_mn[2], G.gm.Z);
+  float d;
+  if (G.mc_nearest) {
+    const int x = sat_f2i(rintf(fx)), y = sat_f2i(rintf(fy)), z = sat_f2i(rintf(fz));
+    d = in_grid<float>(x, y, z, G.gm) ? G.mc[x * G.gm.sx + y * G.gm.sy + z * G.gm.sz] : 0.f;
+  } else {
+    Tri<float> t;
+    t.init(fx, fy, fz);
+    d = tri_gather(G.mc, G.gm, t, 0);
+  }
+  const float x = d + G.mc_shift;
+  const float sp = x > 20.f ? x : log1pf(expf(x));
+  return 1.f - expf(-sp * G.mc_ratio) >= G.mc_thres;
+}
 
 __device__ __forceinline__ uint64_t lanemask_lt(int lane) {
   return lane ? ((~0ull) >> (64 - lane)) : 0ull;
@@ -48,6 +73,7 @@ __device__ __forceinline__ void run_chunk(const FusedGeom& G, const float* start
   bool out = true;
   if (valid) out = sample_point<float>(start, dir, G.mn, G.mx, G.stepdist, s, cs.q[0], cs.q[1], cs.q[2]);
   cs.inb = valid && !out;
+  if (G.mc && cs.inb) cs.inb = maskcache_pass(G, cs.q);
   cs.alpha = 0.f;
   cs.sdf = 0.f;
   cs.grad[0] = cs.grad[1] = cs.grad[2] = 0.f;
@@ -416,18 +442,47 @@ static int fused_geom(FusedGeom& G, const float* sdf, const int64_t* sdf_geom, c
   G.thres = params[3];
   G.near_ = params[4];
   G.far_ = params[5];
+  G.mc = nullptr;
+  return VX_OK;
+}
+
+// mc_params = {act_shift, voxel_size_ratio, thres, nearest, xyz_min[3], xyz_max[3]}
+static int fused_maskcache(FusedGeom& G, const float* mc, const int64_t* mc_geom,
+                           const float* mc_params) {
+  if (!mc) return VX_OK;
+  VX_CHECK_ARG(mc_geom && mc_params, "fused: mask cache geometry/parameters missing");
+  G.mc = mc;
+  G.gm.C = (int)mc_geom[0];
+  G.gm.X = (int)mc_geom[1];
+  G.gm.Y = (int)mc_geom[2];
+  G.gm.Z = (int)mc_geom[3];
+  G.gm.sc = mc_geom[4];
+  G.gm.sx = mc_geom[5];
+  G.gm.sy = mc_geom[6];
+  G.gm.sz = mc_geom[7];
+  VX_CHECK_ARG(G.gm.C == 1, "fused: the mask-cache density must have one channel");
+  G.mc_shift = mc_params[0];
+  G.mc_ratio = mc_params[1];
+  G.mc_thres = mc_params[2];
+  G.mc_nearest = mc_params[3] != 0.f;
+  for (int i = 0; i < 3; ++i) {
+    G.mc_mn[i] = mc_params[4 + i];
+    G.mc_mx[i] = mc_params[7 + i];
+  }
   return VX_OK;
 }
 
 extern "C" {
 
 int vx_fused_fine_fwd(const float* sdf, const int64_t* sdf_geom, const float* box,
-                      const float* params, const float* rays_o, const float* rays_d,
+                      const float* params, const float* mc, const int64_t* mc_geom,
+                      const float* mc_params, const float* rays_o, const float* rays_d,
                       const float* viewdirs, int64_t n_rays, int cap, void* s4, void* g4,
                       int64_t* cnt, float* last, int* stats, int count_all, int* overflow,
                       void* stream) {
   FusedGeom G;
   if (int st = fused_geom(G, sdf, sdf_geom, nullptr, nullptr, box, params)) return st;
+  if (int st = fused_maskcache(G, mc, mc_geom, mc_params)) return st;
   VX_CHECK_ARG(n_rays >= 0 && cap >= 1, "fused_fine_fwd: bad sizes");
   if (n_rays == 0) return VX_OK;
   k_fused_fwd<<<grid_for(n_rays, 4, 1 << 16), 256, 0, (hipStream_t)stream>>>(
@@ -478,13 +533,15 @@ int vx_fused_fine_feat_bwd(const int64_t* k0_geom, const int64_t* sdf_geom, cons
 }
 
 int vx_fused_fine_bwd(const float* sdf, const int64_t* sdf_geom, const float* box,
-                      const float* params, const float* rays_o, const float* rays_d,
+                      const float* params, const float* mc, const int64_t* mc_geom,
+                      const float* mc_params, const float* rays_o, const float* rays_d,
                       const float* viewdirs, int64_t n_rays, const int64_t* cnt,
                       const int64_t* off_incl, const float* last, const float* gw,
                       const float* glast, const float* S, const float* dgf, float* sdf_grad,
                       void* stream) {
   FusedGeom G;
   if (int st = fused_geom(G, sdf, sdf_geom, nullptr, nullptr, box, params)) return st;
+  if (int st = fused_maskcache(G, mc, mc_geom, mc_params)) return st;
   if (n_rays == 0) return VX_OK;
   k_fused_bwd<<<grid_for(n_rays, 4, 1 << 16), 256, 0, (hipStream_t)stream>>>(
       G, rays_o, rays_d, viewdirs, n_rays, cnt, off_incl, last, gw, glast, S, dgf, sdf_grad);

@@ -471,6 +471,11 @@ extern "C" {
 const char* vx_last_error(void) { return vx::g_err; }
 int vx_version(void) { return 1; }
 
+#ifndef VX_ABI_HASH
+#define VX_ABI_HASH "unknown"
+#endif
+const char* vx_abi_hash(void) { return VX_ABI_HASH; }
+
 int vx_infer_t_minmax(int dtype, const void* rays_o, const void* rays_d, const void* xyz_min,
                       const void* xyz_max, float near_, float far_, int64_t n, void* t_min,
                       void* t_max, void* stream) {

@@ -31,6 +31,20 @@ from .total_variation_cuda import grid_geom
 XSTRIDE = 48
 
 
+def maskcache_args(mc):
+    """Arguments of the fused kernels for an optional voxurf_amd.maskcache.MaskCache."""
+    if mc is None:
+        return (None, None, None)
+    if not hasattr(mc, "_fused_cache") or mc._fused_cache[0] is not mc.density:
+        geom = grid_geom(mc.density)
+        params = torch.tensor([mc.act_shift, mc.voxel_size_ratio, mc.mask_cache_thres,
+                               1.0 if mc.nearest else 0.0, *mc.xyz_min.tolist(),
+                               *mc.xyz_max.tolist()], dtype=torch.float32)
+        mc._fused_cache = (mc.density, geom, params)
+    d, geom, params = mc._fused_cache
+    return (ptr(d), geom.data_ptr(), params.data_ptr())
+
+
 class FusedFineTrainer:
     def __init__(self, cfg, sdf, k0, mlp_params=None, device="cuda:0", mlp_dtype=torch.float32,
                  lr_scale=1.0):
@@ -62,6 +76,12 @@ class FusedFineTrainer:
         self.last_profile = {}
         self.last_counts = {}
         self.count_all = True
+
+    mask_cache = None
+
+    def _mc_args(self):
+        """(density ptr, geom ptr, params ptr) of the optional MaskCache, or NULLs."""
+        return maskcache_args(self.mask_cache)
 
     # --- buffers ------------------------------------------------------------------
     def _ensure(self, n):
@@ -106,7 +126,7 @@ class FusedFineTrainer:
         self._ensure(n)
         st = stream_of(self.sdf)
         call("vx_fused_fine_fwd", ptr(self.sdf), self.sdf_geom.data_ptr(), self.box.data_ptr(),
-             self.params.data_ptr(), ptr(rays_o), ptr(rays_d), ptr(viewdirs), n, self.cap,
+             self.params.data_ptr(), *self._mc_args(), ptr(rays_o), ptr(rays_d), ptr(viewdirs), n, self.cap,
              ptr(self.s4), ptr(self.g4), ptr(self.cnt), ptr(self.last), ptr(self.stats),
              int(self.count_all), ptr(self.overflow), st)
         call("vx_inclusive_scan_i64", ptr(self.cnt), ptr(self.off), n, ptr(self.ws), st)
@@ -150,7 +170,7 @@ class FusedFineTrainer:
              self.box.data_ptr(), self.params.data_ptr(), m, ptr(dX), dX.stride(0), ptr(self.P4),
              ptr(self.G4), ptr(self.k0_grad), ptr(dgf), ptr(self.k0_pages), st)
         call("vx_fused_fine_bwd", ptr(self.sdf), self.sdf_geom.data_ptr(), self.box.data_ptr(),
-             self.params.data_ptr(), ptr(rays_o), ptr(rays_d), ptr(viewdirs), n, ptr(self.cnt),
+             self.params.data_ptr(), *self._mc_args(), ptr(rays_o), ptr(rays_d), ptr(viewdirs), n, ptr(self.cnt),
              ptr(self.off), ptr(self.last), ptr(gw), ptr(self.glast), ptr(self.S), ptr(dgf),
              ptr(self.sdf_grad), st)
         if profile:

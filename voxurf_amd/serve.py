@@ -27,7 +27,7 @@ import torch
 
 from . import _lib
 from ._lib import call, ptr, stream_of
-from .fused import XSTRIDE
+from .fused import XSTRIDE, maskcache_args
 from .mlp import RgbNet
 from .total_variation_cuda import grid_geom
 
@@ -80,6 +80,11 @@ class ResidentScene:
         self.xyz_min = torch.tensor([-1.0, -1.0, -1.0], device=self.dev)
         self.xyz_max = torch.tensor([1.0, 1.0, 1.0], device=self.dev)
 
+    mask_cache = None
+
+    def _mc_args(self):
+        return maskcache_args(self.mask_cache)
+
     def nbytes(self):
         n = self.sdf.numel() + self.k0.numel() + sum(p.numel() for p in self.rgbnet.parameters())
         return 4 * n
@@ -95,7 +100,7 @@ class ResidentScene:
         ws.ensure(n)
         st = stream_of(rays_o)
         call("vx_fused_fine_fwd", ptr(self.sdf), self.sdf_geom.data_ptr(), self.box.data_ptr(),
-             self.params.data_ptr(), ptr(rays_o), ptr(rays_d), ptr(viewdirs), n, self.cap,
+             self.params.data_ptr(), *self._mc_args(), ptr(rays_o), ptr(rays_d), ptr(viewdirs), n, self.cap,
              ptr(ws.s4), ptr(ws.g4), ptr(ws.cnt), ptr(ws.last), ptr(ws.stats), 0, ptr(ws.overflow), st)
         call("vx_inclusive_scan_i64", ptr(ws.cnt), ptr(ws.off), n, ptr(ws.ws), st)
         m = int(ws.off[n - 1].item())
