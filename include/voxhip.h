@@ -296,18 +296,19 @@ int vx_mlp_layer_fwd(int bf16, int kt, int last, const void* in, int in_stride, 
 int vx_mlp_layer_bwd(int bf16, int nto, const float* dy, int dy_stride, int64_t M,
                      const void* wpk_t, const void* mask, int mask_stride, float* dx,
                      int dx_stride, void* stream);
-/* dH = (dlogit W3) * [H > 0] for the 3-output layer. */
-int vx_mlp_dh_last(int bf16, const float* dlogit, const void* H, int h_stride, int64_t M,
-                   const float* w3, float* dH, int dh_stride, void* stream);
-/* Row blocks of the weight-gradient kernels for M rows (partial buffers hold that many
- * [192 x 32*kt] slices). */
-int vx_mlp_wgrad_blocks(int64_t M);
-/* dW [192, k_real] = dY^T X, db [192] = column sums of dY (split over row blocks). */
+/* Floats the `part` / `partb` workspaces of vx_mlp_wgrad and vx_mlp_last_bwd need for
+ * M rows (row-block partial sums plus the reduction's group sums). */
+int64_t vx_mlp_wgrad_part_elems(int64_t M);
+int64_t vx_mlp_wgrad_partb_elems(int64_t M);
+/* dW [192, k_real] = dY^T X, db [192] = column sums of dY (split over row blocks, then
+ * a deterministic two-pass reduction). */
 int vx_mlp_wgrad(int bf16, int kt, int k_real, const float* dY, int dys, const void* X, int xs,
                  int64_t M, float* part, float* partb, float* dW, float* db, void* stream);
-/* dW3 [3, 192] = dlogit^T H, db3 [3] = column sums of dlogit. */
-int vx_mlp_wgrad_last(int bf16, const float* dlogit, const void* H, int hs, int64_t M,
-                      float* part, float* partb, float* dW3, float* db3, void* stream);
+/* Backward through the 3-output layer in one pass over H: dH = (dlogit W3) * [H > 0],
+ * dW3 [3, 192] = dlogit^T H, db3 [3] = column sums of dlogit. */
+int vx_mlp_last_bwd(int bf16, const float* dlogit, const void* H, int hs, int64_t M,
+                    const float* w3, float* dH, int dhs, float* part, float* partb, float* dW3,
+                    float* db3, void* stream);
 /* y = bfloat16(x), n elements. */
 int vx_f32_to_bf16(const float* x, int64_t n, void* y, void* stream);
 

@@ -1,13 +1,19 @@
 """Differentiation checks of the hand-written backward kernels (GPU, float64).
 
 Scene: the coarse scene (96^3 analytic SDF + RGB grid, s = 64), 64 rays.
-  * finite differences: for 32 grid entries (16 SDF voxels and 16 colour entries, drawn
-    among entries the batch touches), (L(p + eps) - L(p - eps)) / (2 eps) with
-    eps = 1e-6 against the analytic gradient from the HIP backward, all in float64;
-    each within 1e-4 relative (floor 1e-9);
+  * finite differences, (L(p + eps) - L(p - eps)) / (2 eps) against the analytic
+    gradient from the HIP backward, all in float64:
+      - 16 colour entries drawn among the entries the batch touches, eps = 1e-6, each
+        within 1e-4 relative (floor 1e-9);
+      - 16 SDF entries drawn among those whose gradient exceeds 2% of the largest,
+        eps = 1e-3, each within 5e-3 relative.  The reference's alpha2weight forward
+        keeps the transmittance in a float accumulator even for double tensors
+        (render_utils_kernel.cu:589-596), so the loss is a staircase in T at float
+        resolution: a smaller step, or an entry with a small gradient, measures that
+        rounding rather than the derivative (the CPU restatement shows the same);
   * the float64 gradients against autograd of the CPU restatement (torch_ref), within
-    1e-6 relative of the batch maximum (the reference's alpha2weight adjoint keeps a
-    float accumulator even for double tensors, render_utils_kernel.cu:671).
+    1e-6 relative of the batch maximum (the reference's alpha2weight adjoint also keeps a
+    float accumulator, render_utils_kernel.cu:671).
 Samples whose alpha or weight lies within the perturbation of the 1e-4 compaction
 thresholds would make the loss non-differentiable there; at most 2 of the 32 entries
 may miss the bar for that reason (their values are printed).
@@ -47,24 +53,26 @@ def test_finite_differences_float64(dev):
     gs, gk = model.sdf.grid.grad.clone(), model.k0.grid.grad.clone()
     rng = np.random.default_rng(0)
     picks = []
-    for name, g in (("sdf", gs), ("k0", gk)):
-        nz = torch.nonzero(g.flatten().abs() > 1e-6 * float(g.abs().max())).flatten().cpu().numpy()
+    for name, g, frac in (("sdf", gs, 2e-2), ("k0", gk, 1e-6)):
+        nz = torch.nonzero(g.flatten().abs() > frac * float(g.abs().max())).flatten().cpu().numpy()
         for i in rng.choice(nz, 16, replace=False):
             picks.append((name, int(i)))
-    eps = 1e-6
     bad = []
     with torch.no_grad():
         for name, i in picks:
-            p = (model.sdf.grid if name == "sdf" else model.k0.grid).view(-1)
-            p0 = float(p[i])
-            p[i] = p0 + eps
+            # logical (N, C, X, Y, Z) index: the feature grid is stored channels-last
+            p = model.sdf.grid if name == "sdf" else model.k0.grid
+            eps, rtol = (1e-3, 5e-3) if name == "sdf" else (1e-6, 1e-4)
+            at = tuple(int(j) for j in np.unravel_index(i, tuple(p.shape)))
+            p0 = float(p[at])
+            p[at] = p0 + eps
             lp = float(_loss(model, O, D, V, TG))
-            p[i] = p0 - eps
+            p[at] = p0 - eps
             lm = float(_loss(model, O, D, V, TG))
-            p[i] = p0
+            p[at] = p0
             fd = (lp - lm) / (2 * eps)
-            an = float((gs if name == "sdf" else gk).view(-1)[i])
-            ok = abs(fd - an) <= 1e-4 * abs(an) + 1e-9
+            an = float((gs if name == "sdf" else gk)[at])
+            ok = abs(fd - an) <= rtol * abs(an) + 1e-9
             print(f"{name}[{i}] analytic {an:+.10e} fd {fd:+.10e} {'ok' if ok else 'MISS'}")
             if not ok:
                 bad.append((name, i, an, fd))

@@ -95,7 +95,10 @@ def test_fine_step_matches_restatement(dev, setup, path):
         # whose gradient is within rounding noise of zero may move differently from the
         # restatement.  The optimiser is therefore checked against utils.Adam applied
         # to this path's own gradients (1e-5), and against the restatement's parameters
-        # only where the gradient is well above its noise floor.
+        # only where the gradient is well above its noise floor: the step's sensitivity
+        # to g is lr * eps / (|g| + eps)^2, so a 1e-4 relative gradient difference moves
+        # the parameter by up to lr * 2.5e-5 when |g| ~ eps; from |g| > 1e3 * eps on it
+        # is below lr * 1e-7.
         pairs = [("sdf", st["sdf"], grads["sdf"], sdf_after, CFG.lr_sdf, ref_after["sdf"], ref["grads"]["sdf"]),
                  ("k0", st["k0"], grads["k0"], k0_after, CFG.lr_k0, ref_after["k0"], ref["grads"]["k0"])]
         flat0 = [x for wb in st["mlp"] for x in wb]
@@ -107,7 +110,7 @@ def test_fine_step_matches_restatement(dev, setup, path):
             tr.adam_step(exp, g.detach().cpu().double(), torch.zeros_like(exp), torch.zeros_like(exp),
                          1, lr, 0.9, 0.99, 1e-8)
             close(p1, exp, 1e-5, 1e-7, name + " after Adam (own gradient)")
-            big = gref.abs() > 1e-3 * float(gref.abs().max())
+            big = (gref.abs() > 1e-3 * float(gref.abs().max())) & (gref.abs() > 1e-5)
             close(p1.detach().cpu()[big.cpu()], q1[big], 1e-5, 1e-6, name + " after Adam vs restatement")
 
 

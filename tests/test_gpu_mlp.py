@@ -2,9 +2,13 @@
 
 FP32 path: logits within 1e-5 relative of a float64 CPU reference (floor 1e-6 of the
 batch maximum), input/weight/bias gradients within 1e-4 relative (floor 1e-5).
-BF16 path (FP32 accumulation, BF16 activations): logits within 3e-2 of the largest
-logit, gradients within 5e-2 of the largest gradient entry, against float64; the
-measured errors are printed.
+BF16 path (BF16 MFMA operands, FP32 accumulation, BF16 stored activations): against a
+float64 emulation that rounds to bfloat16 at exactly the points the kernels do (the
+network input, the packed weights, each stored activation, the incoming gradient of each
+data-gradient GEMM) every output and gradient is within 5e-3 of its largest entry (the
+emulation cannot follow the FP32 summation order, so an activation near a bfloat16
+rounding boundary can land one bfloat16 step away) -- a layout or indexing mistake gives
+O(1) errors; the distance to plain float64 is printed (logits within 3e-2).
 """
 import pytest
 import torch
@@ -23,6 +27,35 @@ def ref_forward(x, net):
         if i < len(lins) - 1:
             h = torch.relu(h)
     return h
+
+
+def bf(t):
+    return t.to(torch.bfloat16).to(torch.float64)
+
+
+def emulate_bf16(x, net, g):
+    """float64 model of the BF16 kernels: logits and every gradient."""
+    lins = net.linears()
+    W = [l.weight.detach().double().cpu() for l in lins]
+    b = [l.bias.detach().double().cpu() for l in lins]
+    h = bf(x.detach().double().cpu())
+    g = g.double().cpu()
+    hs = [h]
+    for i in range(len(lins) - 1):
+        a = torch.relu(h @ bf(W[i]).T + b[i])
+        h = bf(a)
+        hs.append(h)
+    logits = hs[-1] @ W[-1].T + b[-1]
+    grads = [None] * (2 * len(lins))
+    grads[-2], grads[-1] = g.T @ hs[-1], g.sum(0)
+    dh = (g @ W[-1]) * (hs[-1] > 0)
+    for li in range(len(lins) - 2, -1, -1):
+        grads[2 * li], grads[2 * li + 1] = dh.T @ hs[li], dh.sum(0)
+        d = bf(dh) @ bf(W[li])
+        if li > 0:
+            d = d * (hs[li] > 0)
+        dh = d
+    return logits, dh, grads
 
 
 def rel_err(a, b):
@@ -60,9 +93,16 @@ def test_rgbnet_mfma(dev, din, M, dtype):
                                        msg=lambda m: f"{n}: {m}")
     else:
         e = rel_err(out, o_r.detach())
-        print(f"bf16 logits rel err {e:.3g}")
+        print(f"bf16 logits vs float64: rel err {e:.3g}")
         assert e < 3e-2
         for n, a, b in pairs:
+            print(f"bf16 grad {n} vs float64: rel err {rel_err(a, b):.3g}")
+        em_logits, em_dx, em_grads = emulate_bf16(x[:, :din], net, g)
+        e = rel_err(out, em_logits)
+        print(f"bf16 logits vs emulation: rel err {e:.3g}")
+        assert e < 5e-3
+        names = ["x"] + [n for n, _ in net.named_parameters()]
+        for n, a, b in zip(names, [x.grad] + [p.grad for p in net.parameters()], [em_dx] + em_grads):
             e = rel_err(a, b)
-            print(f"bf16 grad {n} rel err {e:.3g}")
-            assert e < 5e-2, n
+            print(f"bf16 grad {n} vs emulation: rel err {e:.3g}")
+            assert e < 5e-3, n

@@ -296,44 +296,55 @@ __device__ __forceinline__ void mark_page(uint32_t* pages, int64_t addr) {
   pages[addr >> 10] = 1u;
 }
 
-__global__ void k_fused_feat_bwd(FusedGeom G, int64_t m, const float* __restrict__ dX, int dxs,
-                                 const float4* __restrict__ P4, const float4* __restrict__ G4,
-                                 float* __restrict__ k0_grad, float* __restrict__ dgf,
-                                 uint32_t* __restrict__ pages) {
+__global__ void __launch_bounds__(256) k_fused_feat_bwd(
+    FusedGeom G, int64_t m, const float* __restrict__ dX, int dxs,
+    const float4* __restrict__ P4, const float4* __restrict__ G4, float* __restrict__ k0_grad,
+    float* __restrict__ dgf, uint32_t* __restrict__ pages) {
   const int C = G.gk.C;
-  for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < m;
-       j += (int64_t)gridDim.x * blockDim.x) {
-    const float4 a = P4[j];
-    Tri<float> t;
-    t.init(world_to_index(a.x, G.mn[0], G.mx[0] - G.mn[0], G.gk.X),
-           world_to_index(a.y, G.mn[1], G.mx[1] - G.mn[1], G.gk.Y),
-           world_to_index(a.z, G.mn[2], G.mx[2] - G.mn[2], G.gk.Z));
-    for (int ch = 0; ch < C; ++ch) tri_scatter(k0_grad, G.gk, t, ch, dX[j * dxs + ch]);
-    if (pages) {
-      // flag every page the scatter may have written: per in-grid corner, its channels
-      // (one run of C floats when channels are innermost, C strided words otherwise)
-      for (int bx = 0; bx < 2; ++bx)
-        for (int by = 0; by < 2; ++by)
-          for (int bz = 0; bz < 2; ++bz) {
-            const int x = t.x0 + bx, y = t.y0 + by, z = t.z0 + bz;
-            if (!in_grid<float>(x, y, z, G.gk)) continue;
-            const int64_t a = x * G.gk.sx + y * G.gk.sy + z * G.gk.sz;
-            if (G.gk.sc == 1) {
-              mark_page(pages, a);
-              mark_page(pages, a + C - 1);
-            } else {
-              for (int ch = 0; ch < C; ++ch) mark_page(pages, a + ch * G.gk.sc);
-            }
+  const int lane = threadIdx.x & 63, grp = lane >> 4, cl = lane & 15;
+  const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
+  for (int64_t base = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64; base < m;
+       base += nwaves * 64) {
+    // Trilinear scatter of the feature adjoint: four samples per pass, sixteen lanes
+    // per sample, lane cl taking channels cl, cl + 16, ...  With channels innermost the
+    // lanes of one atomic instruction hit one or two cache lines per sample instead of
+    // one line per lane.
+    for (int i = 0; i < 16; ++i) {
+      const int64_t j = base + 4 * i + grp;
+      if (j >= m) break;
+      const float4 a = P4[j];
+      Tri<float> t;
+      t.init(world_to_index(a.x, G.mn[0], G.mx[0] - G.mn[0], G.gk.X),
+             world_to_index(a.y, G.mn[1], G.mx[1] - G.mn[1], G.gk.Y),
+             world_to_index(a.z, G.mn[2], G.mx[2] - G.mn[2], G.gk.Z));
+      for (int ch = cl; ch < C; ch += 16) tri_scatter(k0_grad, G.gk, t, ch, dX[j * dxs + ch]);
+      if (pages && cl < 8) {
+        // flag every page corner cl's channels may occupy (one run of C floats when
+        // channels are innermost, C strided words otherwise)
+        const int x = t.x0 + (cl >> 2), y = t.y0 + ((cl >> 1) & 1), z = t.z0 + (cl & 1);
+        if (in_grid<float>(x, y, z, G.gk)) {
+          const int64_t o = x * G.gk.sx + y * G.gk.sy + z * G.gk.sz;
+          if (G.gk.sc == 1) {
+            mark_page(pages, o);
+            mark_page(pages, o + C - 1);
+          } else {
+            for (int ch = 0; ch < C; ++ch) mark_page(pages, o + ch * G.gk.sc);
           }
+        }
+      }
     }
-    const float4 b = G4[j];
-    const float gvec[3] = {b.x, b.y, b.z};
-    const float dn[3] = {dX[j * dxs + C], dX[j * dxs + C + 1], dX[j * dxs + C + 2]};
-    const float nr = sqrtf(gvec[0] * gvec[0] + gvec[1] * gvec[1] + gvec[2] * gvec[2]);
-    const float den = nr + 1e-5f;
-    const float dot = gvec[0] * dn[0] + gvec[1] * dn[1] + gvec[2] * dn[2];
-    const float k = nr > 0.f ? dot / (nr * den) : 0.f;
-    for (int i = 0; i < 3; ++i) dgf[3 * j + i] = (dn[i] - gvec[i] * k) / den;
+    // adjoint of the normal n = g / (|g| + 1e-5): one sample per lane
+    const int64_t j = base + lane;
+    if (j < m) {
+      const float4 b = G4[j];
+      const float gvec[3] = {b.x, b.y, b.z};
+      const float dn[3] = {dX[j * dxs + C], dX[j * dxs + C + 1], dX[j * dxs + C + 2]};
+      const float nr = sqrtf(gvec[0] * gvec[0] + gvec[1] * gvec[1] + gvec[2] * gvec[2]);
+      const float den = nr + 1e-5f;
+      const float dot = gvec[0] * dn[0] + gvec[1] * dn[1] + gvec[2] * dn[2];
+      const float k = nr > 0.f ? dot / (nr * den) : 0.f;
+      for (int q = 0; q < 3; ++q) dgf[3 * j + q] = (dn[q] - gvec[q] * k) / den;
+    }
   }
 }
 
@@ -527,7 +538,7 @@ int vx_fused_fine_feat_bwd(const int64_t* k0_geom, const int64_t* sdf_geom, cons
   FusedGeom G;
   if (int st = fused_geom(G, nullptr, sdf_geom, nullptr, k0_geom, box, params)) return st;
   if (m == 0) return VX_OK;
-  k_fused_feat_bwd<<<grid_for(m, 256), 256, 0, (hipStream_t)stream>>>(
+  k_fused_feat_bwd<<<grid_for(m, 256, 1 << 16), 256, 0, (hipStream_t)stream>>>(
       G, m, dX, dxs, (const float4*)P4, (const float4*)G4, k0_grad, dgf, k0_pages);
   return launch_status("fused_fine_feat_bwd");
 }

@@ -2,19 +2,18 @@
 // MLP of lib/voxurf_fine.py:132-149, forward and backward, FP32 (v_mfma_f32_32x32x2f32)
 // and BF16 inputs with FP32 accumulation (v_mfma_f32_32x32x16_bf16).
 //
-// Layout trick.  A 32x32 MFMA leaves output element (row i, col j) in lane
-// j + 32*((i % 8) / 4), register (i % 4) + 4*(i / 8).  The layers here compute
-// H_out^T = W * H_in^T, i.e. the sample index is the MFMA column (one sample per lane
-// pair) and the feature index the row, so after a layer each lane already holds 96 of
-// its sample's 192 output features.  The next layer consumes them in place by
-// permuting its reduction order: reduction step s of lane half h uses input feature
-// dfeat(s / 16, s % 16, h), and the weights are packed once per call in exactly that
-// order (and in lane order, so every LDS read is conflict-free).  Activations therefore
-// never pass through LDS; only the weights do, loaded once per workgroup.
+// Layout.  A 32x32 MFMA leaves output element (row i, col j) in lane
+// j + 32*((i % 8) / 4), register (i % 4) + 4*(i / 8).  The layers compute
+// H_out^T = W * H_in^T: the sample index is the MFMA column and the feature index the
+// row, so a lane pair holds one sample.  The B operand of reduction step s in lane half
+// h is input feature dfeat(s / 16, s % 16, h) of that lane's sample (four consecutive
+// features per float4 load), and the weights are packed once per call in exactly that
+// order and in lane order, so every LDS read of the A operand is one conflict-free
+// ds_read_b128 per four MFMAs.
 //
 // Per 32-sample tile and layer: 6 output tiles x 96 reduction steps = 576 MFMAs (FP32)
 // or 6 x 12 = 72 (BF16).  Weight gradients dW = dY^T X are a separate split-M GEMM
-// through LDS (its reduction runs over samples).
+// whose partial sums are reduced in two deterministic passes.
 #include "common.h"
 
 namespace vx {
@@ -76,33 +75,29 @@ __global__ void k_pack_bf16(const float* __restrict__ Wt, int n_out, int k_in, i
 
 // ---- one layer over all rows -------------------------------------------------------
 // MODE 0: out = relu(W x + b)            (hidden layer, forward)
-// MODE 1: as 0, plus logits = W3 out + b3 (last hidden layer, forward)
 // MODE 2: out = (W^T dy) * [mask > 0]    (backward data through a ReLU layer)
 // MODE 3: out = W^T dy                   (backward data to the network input)
-// Input rows: `in` [M, in_stride] (float32 or bfloat16 per TIN), read at the permuted
-// feature order.  Output rows [M, out_stride] in the same element type as TIN for
-// activations (MODE 0/1) and float32 for gradients (MODE 2/3).
+// Input rows: `in` [M, in_stride], read at the permuted feature order: activations
+// (MODE 0) in the layer's element type, gradients (MODE 2/3) float32.  Output rows
+// [M, out_stride]: activations in the element type (bfloat16 when BF), gradients
+// float32.  Persistent: each wave walks 32-row tiles; weights stay in LDS.
 template <int KT, int NTO, int MODE, bool BF>
 __global__ void __launch_bounds__(512) k_mlp_layer(
     const void* __restrict__ in_, int in_stride, int64_t M, const void* __restrict__ wpk,
     const float* __restrict__ bias, const void* __restrict__ mask_, int mask_stride,
-    void* __restrict__ out_, int out_stride, const float* __restrict__ w3,
-    const float* __restrict__ b3, float* __restrict__ logits) {
+    void* __restrict__ out_, int out_stride) {
   constexpr int S = BF ? 2 * KT : 16 * KT;     // reduction steps
   constexpr int WELEMS = NTO * S * 64 * (BF ? 8 : 1);
   __shared__ __attribute__((aligned(16))) unsigned char lds_raw[WELEMS * (BF ? 2 : 4)];
   __shared__ float lb[NTO * 32];
-  __shared__ float lw3[3 * kWidth];
   const int tid = threadIdx.x;
   {
     const float4* src = reinterpret_cast<const float4*>(wpk);
     float4* dst = reinterpret_cast<float4*>(lds_raw);
     constexpr int n4 = WELEMS * (BF ? 2 : 4) / 16;
     for (int i = tid; i < n4; i += blockDim.x) dst[i] = src[i];
-    if (MODE <= 1)
+    if (MODE == 0)
       for (int i = tid; i < NTO * 32; i += blockDim.x) lb[i] = bias[i];
-    if (MODE == 1)
-      for (int i = tid; i < 3 * kWidth; i += blockDim.x) lw3[i] = w3[i];
   }
   __syncthreads();
   const int lane = tid & 63, wave = tid >> 6, h = lane >> 5, rl = lane & 31;
@@ -117,81 +112,96 @@ __global__ void __launch_bounds__(512) k_mlp_layer(
     for (int t = 0; t < NTO; ++t)
 #pragma unroll
       for (int v = 0; v < 16; ++v) acc[t][v] = 0.f;
+    // rows past M read row M-1 (in bounds); their results are never stored
+    const int64_t rrow = valid ? row : M - 1;
     if constexpr (!BF) {
-      const float* in = reinterpret_cast<const float*>(in_);
-      float b[S];
+      // The B operand of reduction step s = 16 t + 4 u + w is input feature
+      // 32 t + 8 u + 4 h + w of this lane's row: one float4 per (t, u).  The 16 values
+      // of input tile t + 1 are loaded while tile t's 16 * NTO MFMAs run, so at most two
+      // tiles of operands are live beside the NTO accumulators.
+      const float* rp = reinterpret_cast<const float*>(in_) + rrow * in_stride + 4 * h;
+      const float4* lw = reinterpret_cast<const float4*>(lds_raw);
+      float4 cur[4], nxt[4];
 #pragma unroll
-      for (int t = 0; t < KT; ++t)
+      for (int u = 0; u < 4; ++u) cur[u] = *reinterpret_cast<const float4*>(rp + 8 * u);
+#pragma unroll 1
+      for (int t = 0; t < KT; ++t) {
+        if (t + 1 < KT) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            nxt[u] = *reinterpret_cast<const float4*>(rp + 32 * (t + 1) + 8 * u);
+        }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-          float4 f = make_float4(0.f, 0.f, 0.f, 0.f);
-          if (valid) f = *reinterpret_cast<const float4*>(in + row * in_stride + 32 * t + 8 * u + 4 * h);
-          b[t * 16 + 4 * u + 0] = f.x;
-          b[t * 16 + 4 * u + 1] = f.y;
-          b[t * 16 + 4 * u + 2] = f.z;
-          b[t * 16 + 4 * u + 3] = f.w;
+          const int sq = 4 * t + u;
+#pragma unroll
+          for (int to = 0; to < NTO; ++to) {
+            const float4 a = lw[(to * (S / 4) + sq) * 64 + lane];
+            acc[to] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, cur[u].x, acc[to], 0, 0, 0);
+            acc[to] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.y, cur[u].y, acc[to], 0, 0, 0);
+            acc[to] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.z, cur[u].z, acc[to], 0, 0, 0);
+            acc[to] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.w, cur[u].w, acc[to], 0, 0, 0);
+          }
         }
-      const float4* lw = reinterpret_cast<const float4*>(lds_raw);
+        if (t + 1 < KT) {
 #pragma unroll
-      for (int sq = 0; sq < S / 4; ++sq) {
-#pragma unroll
-        for (int to = 0; to < NTO; ++to) {
-          const float4 a = lw[(to * (S / 4) + sq) * 64 + lane];
-          acc[to] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, b[4 * sq + 0], acc[to], 0, 0, 0);
-          acc[to] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.y, b[4 * sq + 1], acc[to], 0, 0, 0);
-          acc[to] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.z, b[4 * sq + 2], acc[to], 0, 0, 0);
-          acc[to] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.w, b[4 * sq + 3], acc[to], 0, 0, 0);
+          for (int u = 0; u < 4; ++u) cur[u] = nxt[u];
         }
       }
     } else {
       // BF16: step s takes the 8 features dfeat(s/2, 8 (s%2) + e, h), e = 0..7, which
-      // are 8 consecutive registers of the producing layer's output.
-      bf16x8 b[S];
-      if (MODE <= 1) {
-        const __bf16* in = reinterpret_cast<const __bf16*>(in_);
+      // are 8 consecutive registers of the producing layer's output.  Input tile t
+      // (steps 2t, 2t+1) is loaded while tile t-1's MFMAs run.
+      const bf16x8* lw = reinterpret_cast<const bf16x8*>(lds_raw);
+      bf16x8 cur[2], nxt[2];
+      auto load_tile = [&](bf16x8* dst, int t) {
+        if constexpr (MODE == 0) {
+          // features 32 t + 8 u + 4 h + w for u = 0..3, w = 0..3, stored as bfloat16
+          const __bf16* in = reinterpret_cast<const __bf16*>(in_) + rrow * in_stride + 4 * h + 32 * t;
 #pragma unroll
-        for (int s = 0; s < S; ++s) {
-          // features 32 t + 8 u + 4 h + w for u in {2 (s%2), 2 (s%2) + 1}, w = 0..3
-          const int t = s >> 1, u0 = 2 * (s & 1);
-          uint2 lo = make_uint2(0, 0), hi = make_uint2(0, 0);
-          if (valid) {
-            lo = *reinterpret_cast<const uint2*>(in + row * in_stride + 32 * t + 8 * u0 + 4 * h);
-            hi = *reinterpret_cast<const uint2*>(in + row * in_stride + 32 * t + 8 * (u0 + 1) + 4 * h);
+          for (int q = 0; q < 2; ++q) {
+            const uint2 lo = *reinterpret_cast<const uint2*>(in + 16 * q);
+            const uint2 hi = *reinterpret_cast<const uint2*>(in + 16 * q + 8);
+            uint4 v = make_uint4(lo.x, lo.y, hi.x, hi.y);
+            dst[q] = *reinterpret_cast<bf16x8*>(&v);
           }
-          uint4 q = make_uint4(lo.x, lo.y, hi.x, hi.y);
-          b[s] = *reinterpret_cast<bf16x8*>(&q);
+        } else {
+          // backward: the incoming gradient rows are float32; rounded to bf16 here
+          const float* in = reinterpret_cast<const float*>(in_) + rrow * in_stride + 4 * h + 32 * t;
+#pragma unroll
+          for (int q = 0; q < 2; ++q) {
+            const float4 lo = *reinterpret_cast<const float4*>(in + 16 * q);
+            const float4 hi = *reinterpret_cast<const float4*>(in + 16 * q + 8);
+            dst[q][0] = (__bf16)lo.x;
+            dst[q][1] = (__bf16)lo.y;
+            dst[q][2] = (__bf16)lo.z;
+            dst[q][3] = (__bf16)lo.w;
+            dst[q][4] = (__bf16)hi.x;
+            dst[q][5] = (__bf16)hi.y;
+            dst[q][6] = (__bf16)hi.z;
+            dst[q][7] = (__bf16)hi.w;
+          }
         }
-      } else {
-        // backward: the incoming gradient rows are float32; round to bf16 for the MFMA
-        const float* in = reinterpret_cast<const float*>(in_);
+      };
+      load_tile(cur, 0);
+#pragma unroll 1
+      for (int t = 0; t < KT; ++t) {
+        if (t + 1 < KT) load_tile(nxt, t + 1);
 #pragma unroll
-        for (int s = 0; s < S; ++s) {
-          const int t = s >> 1, u0 = 2 * (s & 1);
-          float4 lo = make_float4(0.f, 0.f, 0.f, 0.f), hi = lo;
-          if (valid) {
-            lo = *reinterpret_cast<const float4*>(in + row * in_stride + 32 * t + 8 * u0 + 4 * h);
-            hi = *reinterpret_cast<const float4*>(in + row * in_stride + 32 * t + 8 * (u0 + 1) + 4 * h);
-          }
-          b[s][0] = (__bf16)lo.x;
-          b[s][1] = (__bf16)lo.y;
-          b[s][2] = (__bf16)lo.z;
-          b[s][3] = (__bf16)lo.w;
-          b[s][4] = (__bf16)hi.x;
-          b[s][5] = (__bf16)hi.y;
-          b[s][6] = (__bf16)hi.z;
-          b[s][7] = (__bf16)hi.w;
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+          for (int to = 0; to < NTO; ++to)
+            acc[to] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lw[(to * S + 2 * t + q) * 64 + lane],
+                                                               cur[q], acc[to], 0, 0, 0);
+        if (t + 1 < KT) {
+          cur[0] = nxt[0];
+          cur[1] = nxt[1];
         }
       }
-      const bf16x8* lw = reinterpret_cast<const bf16x8*>(lds_raw);
-#pragma unroll
-      for (int s = 0; s < S; ++s)
-#pragma unroll
-        for (int to = 0; to < NTO; ++to)
-          acc[to] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lw[(to * S + s) * 64 + lane], b[s],
-                                                             acc[to], 0, 0, 0);
     }
     // ---- epilogue ----
-    float p[3] = {0.f, 0.f, 0.f};
+    const int64_t obase = rrow * out_stride + 4 * h;
+    const int64_t mbase = rrow * mask_stride + 4 * h;
 #pragma unroll
     for (int to = 0; to < NTO; ++to) {
 #pragma unroll
@@ -201,158 +211,107 @@ __global__ void __launch_bounds__(512) k_mlp_layer(
         for (int w = 0; w < 4; ++w) {
           const int v = 4 * u + w;
           float x = acc[to][v];
-          if (MODE <= 1) {
+          if (MODE == 0) {
             x += lb[dfeat(to, v, h)];
             x = x > 0.f ? x : 0.f;
           }
           v4[w] = x;
         }
-        const int64_t col = 32 * to + 8 * u + 4 * h;
+        const int cofs = 32 * to + 8 * u;          // compile-time column offset
+        const int col = cofs + 4 * h;
         if (MODE == 2 && valid) {
           if constexpr (BF) {
-            const __bf16* mk = reinterpret_cast<const __bf16*>(mask_) + row * mask_stride + col;
+            const __bf16* mk = reinterpret_cast<const __bf16*>(mask_) + mbase + cofs;
+#pragma unroll
             for (int w = 0; w < 4; ++w) v4[w] = (float)mk[w] > 0.f ? v4[w] : 0.f;
           } else {
             const float4 mk = *reinterpret_cast<const float4*>(reinterpret_cast<const float*>(mask_) +
-                                                              row * mask_stride + col);
+                                                              mbase + cofs);
             v4[0] = mk.x > 0.f ? v4[0] : 0.f;
             v4[1] = mk.y > 0.f ? v4[1] : 0.f;
             v4[2] = mk.z > 0.f ? v4[2] : 0.f;
             v4[3] = mk.w > 0.f ? v4[3] : 0.f;
           }
         }
-        if (MODE == 1) {
-          // logits from the activations as stored (bf16-rounded in the BF16 path)
-#pragma unroll
-          for (int w = 0; w < 4; ++w) {
-            const float a = BF ? (float)(__bf16)v4[w] : v4[w];
-            const int f = (int)col + w;
-#pragma unroll
-            for (int c = 0; c < 3; ++c) p[c] += a * lw3[c * kWidth + f];
-          }
-        }
         if (valid) {
-          if (BF && MODE <= 1) {
-            __bf16* o = reinterpret_cast<__bf16*>(out_) + row * out_stride + col;
+          if (BF && MODE == 0) {
+            __bf16* o = reinterpret_cast<__bf16*>(out_) + obase + cofs;
             __bf16 q[4] = {(__bf16)v4[0], (__bf16)v4[1], (__bf16)v4[2], (__bf16)v4[3]};
             *reinterpret_cast<uint2*>(o) = *reinterpret_cast<uint2*>(q);
           } else {
-            float* o = reinterpret_cast<float*>(out_) + row * out_stride + col;
+            float* o = reinterpret_cast<float*>(out_) + obase + cofs;
             *reinterpret_cast<float4*>(o) = make_float4(v4[0], v4[1], v4[2], v4[3]);
           }
         }
       }
     }
-    if (MODE == 1) {
-#pragma unroll
-      for (int c = 0; c < 3; ++c) p[c] += __shfl_xor(p[c], 32, 64);
-      if (h == 0 && valid)
-        for (int c = 0; c < 3; ++c) logits[row * 3 + c] = p[c] + b3[c];
-    }
   }
 }
 
-// dH3 = (dlogit W3) * [H3 > 0] (the K = 3 layer, on the vector ALUs).
+// logits [M, 3] = H W3^T + b3 from the stored activations (bfloat16 when BF).  One wave
+// per row: lanes 0..47 take four features each, then a wave sum.
 template <bool BF>
-__global__ void k_mlp_dh_last(const float* __restrict__ dlogit, const void* __restrict__ H_,
-                              int h_stride, int64_t M, const float* __restrict__ w3,
-                              float* __restrict__ dH, int dh_stride) {
-  const int64_t total = M * kWidth;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t r = i / kWidth;
-    const int f = (int)(i % kWidth);
-    const float hv = BF ? (float)reinterpret_cast<const __bf16*>(H_)[r * h_stride + f]
-                        : reinterpret_cast<const float*>(H_)[r * h_stride + f];
-    const float g = dlogit[3 * r] * w3[f] + dlogit[3 * r + 1] * w3[kWidth + f] +
-                    dlogit[3 * r + 2] * w3[2 * kWidth + f];
-    dH[r * dh_stride + f] = hv > 0.f ? g : 0.f;
+__global__ void __launch_bounds__(256) k_mlp_logits(const void* __restrict__ H_, int hs,
+                                                    int64_t M, const float* __restrict__ w3,
+                                                    const float* __restrict__ b3,
+                                                    float* __restrict__ logits) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const bool act = lane < kWidth / 4;
+  const int f = act ? 4 * lane : 0;
+  float4 w[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    w[c] = *reinterpret_cast<const float4*>(w3 + c * kWidth + f);
+    if (!act) w[c] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  const float c0 = b3[0], c1 = b3[1], c2 = b3[2];
+  for (int64_t r = (int64_t)blockIdx.x * 4 + wave; r < M; r += (int64_t)gridDim.x * 4) {
+    float4 x;
+    if constexpr (BF) {
+      const uint2 q = *reinterpret_cast<const uint2*>(reinterpret_cast<const __bf16*>(H_) + r * hs + f);
+      const __bf16* b = reinterpret_cast<const __bf16*>(&q);
+      x = make_float4((float)b[0], (float)b[1], (float)b[2], (float)b[3]);
+    } else {
+      x = *reinterpret_cast<const float4*>(reinterpret_cast<const float*>(H_) + r * hs + f);
+    }
+    float p[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      p[c] = act ? x.x * w[c].x + x.y * w[c].y + x.z * w[c].z + x.w * w[c].w : 0.f;
+      p[c] = wave_sum_f32(p[c]);
+    }
+    if (lane == 0) {
+      logits[3 * r] = p[0] + c0;
+      logits[3 * r + 1] = p[1] + c1;
+      logits[3 * r + 2] = p[2] + c2;
+    }
   }
 }
 
-// ---- weight gradients: part[blk] = dY[rows]^T X[rows], partb[blk] = sum dY[rows] --------
-// dY [M, 192] float32, X [M, 32*KT] (float32 or bf16).  Each block takes a contiguous
-// range of rows; output tiles (tn, tk) of 32x32 are spread over the 4 waves.
-template <int KT, bool BF>
-__global__ void __launch_bounds__(256) k_mlp_wgrad(const float* __restrict__ dY, int dys,
-                                                   const void* __restrict__ X_, int xs, int64_t M,
-                                                   int64_t rows_per_block, float* __restrict__ part,
-                                                   float* __restrict__ partb) {
-  constexpr int K = 32 * KT;
-  constexpr int NTILE = 6 * KT;
-  constexpr int NTW = (NTILE + 3) / 4;
-  __shared__ float sY[32][kWidth];
-  __shared__ float sX[32][K];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, h = lane >> 5, rl = lane & 31;
-  f32x16 acc[NTW];
-#pragma unroll
-  for (int q = 0; q < NTW; ++q)
-#pragma unroll
-    for (int v = 0; v < 16; ++v) acc[q][v] = 0.f;
-  float bacc = 0.f;
-  const int64_t r_begin = blockIdx.x * rows_per_block;
-  const int64_t r_end = (r_begin + rows_per_block < M) ? r_begin + rows_per_block : M;
-  for (int64_t r0 = r_begin; r0 < r_end; r0 += 32) {
-    for (int i = tid; i < 32 * kWidth; i += blockDim.x) {
-      const int rr = i / kWidth, c = i % kWidth;
-      sY[rr][c] = (r0 + rr < r_end) ? dY[(r0 + rr) * dys + c] : 0.f;
-    }
-    for (int i = tid; i < 32 * K; i += blockDim.x) {
-      const int rr = i / K, c = i % K;
-      float v = 0.f;
-      if (r0 + rr < r_end)
-        v = BF ? (float)reinterpret_cast<const __bf16*>(X_)[(r0 + rr) * xs + c]
-               : reinterpret_cast<const float*>(X_)[(r0 + rr) * xs + c];
-      sX[rr][c] = v;
-    }
-    __syncthreads();
-    if (tid < kWidth)
-      for (int rr = 0; rr < 32; ++rr) bacc += sY[rr][tid];
-#pragma unroll
-    for (int j = 0; j < 16; ++j) {
-#pragma unroll
-      for (int q = 0; q < NTW; ++q) {
-        const int tile = wave + 4 * q;
-        if (tile < NTILE) {
-          const int tn = tile / KT, tk = tile % KT;
-          acc[q] = __builtin_amdgcn_mfma_f32_32x32x2f32(sY[2 * j + h][32 * tn + rl],
-                                                        sX[2 * j + h][32 * tk + rl], acc[q], 0, 0, 0);
-        }
-      }
-    }
-    __syncthreads();
-  }
-  float* pb = part + (int64_t)blockIdx.x * kWidth * K;
-#pragma unroll
-  for (int q = 0; q < NTW; ++q) {
-    const int tile = wave + 4 * q;
-    if (tile < NTILE) {
-      const int tn = tile / KT, tk = tile % KT;
-#pragma unroll
-      for (int v = 0; v < 16; ++v) {
-        const int n = 32 * tn + 8 * (v >> 2) + 4 * h + (v & 3);
-        pb[n * K + 32 * tk + rl] = acc[q][v];
-      }
-    }
-  }
-  if (tid < kWidth) partb[(int64_t)blockIdx.x * kWidth + tid] = bacc;
-}
-
-// dW3 [3, 192] and db3 [3] partials over row blocks: part3[blk][c*192 + f], partb3[blk][c].
+// Backward through the 3-output layer, one pass over H:
+//   dH [M, 192] = (dlogit W3) * [H > 0],
+//   part3[blk][c * 192 + f] = sum over the block's rows of dlogit[r][c] H[r][f],
+//   partb3[blk][c]          = sum over the block's rows of dlogit[r][c].
 template <bool BF>
-__global__ void __launch_bounds__(192) k_mlp_wgrad_last(const float* __restrict__ dlogit,
-                                                        const void* __restrict__ H_, int hs,
-                                                        int64_t M, int64_t rows_per_block,
-                                                        float* __restrict__ part3,
-                                                        float* __restrict__ partb3) {
+__global__ void __launch_bounds__(192) k_mlp_last_bwd(const float* __restrict__ dlogit,
+                                                      const void* __restrict__ H_, int hs,
+                                                      int64_t M, int64_t rows_per_block,
+                                                      const float* __restrict__ w3,
+                                                      float* __restrict__ dH, int dhs,
+                                                      float* __restrict__ part3,
+                                                      float* __restrict__ partb3) {
   const int f = threadIdx.x;
+  const float w0 = w3[f], w1 = w3[kWidth + f], w2 = w3[2 * kWidth + f];
   const int64_t r_begin = blockIdx.x * rows_per_block;
   const int64_t r_end = (r_begin + rows_per_block < M) ? r_begin + rows_per_block : M;
   float a0 = 0.f, a1 = 0.f, a2 = 0.f, c0 = 0.f, c1 = 0.f, c2 = 0.f;
+#pragma unroll 4
   for (int64_t r = r_begin; r < r_end; ++r) {
     const float hv = BF ? (float)reinterpret_cast<const __bf16*>(H_)[r * hs + f]
                         : reinterpret_cast<const float*>(H_)[r * hs + f];
     const float g0 = dlogit[3 * r], g1 = dlogit[3 * r + 1], g2 = dlogit[3 * r + 2];
+    const float g = g0 * w0 + g1 * w1 + g2 * w2;
+    dH[r * dhs + f] = hv > 0.f ? g : 0.f;
     a0 += g0 * hv;
     a1 += g1 * hv;
     a2 += g2 * hv;
@@ -371,17 +330,183 @@ __global__ void __launch_bounds__(192) k_mlp_wgrad_last(const float* __restrict_
   }
 }
 
-// out[i] = sum_b part[b * n + i]   (i < n_valid_cols per row: the first k_real columns of
-// each row of width k_pad are kept)
-__global__ void k_reduce_parts(const float* __restrict__ part, int nblk, int rows, int k_pad,
-                               int k_real, float* __restrict__ out) {
-  const int n = rows * k_real;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const int r = i / k_real, c = i % k_real;
-    float s = 0.f;
-    for (int b = 0; b < nblk; ++b) s += part[(int64_t)b * rows * k_pad + r * k_pad + c];
-    out[i] = s;
+// ---- weight gradients: part[blk] = dY[rows]^T X[rows], partb[blk] = sum dY[rows] --------
+// dY [M, 192] float32, X [M, 32*KT] (float32 or bfloat16).  A block takes a contiguous
+// range of rows and streams it through LDS in chunks of 32 rows (double buffered: the
+// next chunk's global loads are in flight while the current chunk's MFMAs run).  Wave w
+// owns output tiles n in [TNW (w % NWN), +TNW) x k in [TKW (w / NWN), +TKW) of the
+// [192, 32 KT] slice.  One MFMA step reduces over two rows: lane half h takes row
+// 2 s + h of the chunk, lane l % 32 the column.
+template <int KT>
+struct WgradShape {
+  // KT = 6: 4 waves of 3x3 tiles; KT = 3: 3 waves of 2x3; KT = 2: 4 waves of 3x1
+  static constexpr int TNW = KT == 3 ? 2 : 3;
+  static constexpr int TKW = KT == 2 ? 1 : 3;
+  static constexpr int NWN = 6 / TNW;
+  static constexpr int NWK = KT / TKW;
+  static constexpr int WAVES = NWN * NWK;
+};
+constexpr int kWgradChunk = 32;
+
+template <int KT, bool BF>
+__global__ void __launch_bounds__(256) k_mlp_wgrad(const float* __restrict__ dY, int dys,
+                                                   const void* __restrict__ X_, int xs, int64_t M,
+                                                   int64_t rows_per_block, float* __restrict__ part,
+                                                   float* __restrict__ partb) {
+  using Sh = WgradShape<KT>;
+  constexpr int K = 32 * KT, TNW = Sh::TNW, TKW = Sh::TKW, NWN = Sh::NWN;
+  constexpr int NT = 64 * Sh::WAVES;
+  constexpr int CH = kWgradChunk;
+  constexpr int NY4 = CH * kWidth / 4 / NT;  // float4 of dY per thread per chunk
+  constexpr int NX4 = CH * K / 4 / NT;       // 4-element groups of X per thread per chunk
+  static_assert(CH * kWidth / 4 % NT == 0 && CH * K / 4 % NT == 0, "chunk split");
+  __shared__ __attribute__((aligned(16))) float sY[2][CH][kWidth];
+  __shared__ __attribute__((aligned(16))) float sX[2][CH][K];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, h = lane >> 5, rl = lane & 31;
+  const int tn0 = TNW * (wave % NWN), tk0 = TKW * (wave / NWN);
+  const int64_t r_begin = blockIdx.x * rows_per_block;
+  const int64_t r_end = (r_begin + rows_per_block < M) ? r_begin + rows_per_block : M;
+
+  float4 ry[NY4], rx[NX4];
+  auto load = [&](int64_t r0) {
+#pragma unroll
+    for (int q = 0; q < NY4; ++q) {
+      const int e = tid + q * NT, row = e / (kWidth / 4), c4 = e % (kWidth / 4);
+      const int64_t r = r0 + row;
+      ry[q] = r < r_end ? *reinterpret_cast<const float4*>(dY + r * dys + 4 * c4)
+                        : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int q = 0; q < NX4; ++q) {
+      const int e = tid + q * NT, row = e / (K / 4), c4 = e % (K / 4);
+      const int64_t r = r0 + row;
+      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (r < r_end) {
+        if constexpr (BF) {
+          const uint2 u = *reinterpret_cast<const uint2*>(reinterpret_cast<const __bf16*>(X_) + r * xs + 4 * c4);
+          const __bf16* b = reinterpret_cast<const __bf16*>(&u);
+          v = make_float4((float)b[0], (float)b[1], (float)b[2], (float)b[3]);
+        } else {
+          v = *reinterpret_cast<const float4*>(reinterpret_cast<const float*>(X_) + r * xs + 4 * c4);
+        }
+      }
+      rx[q] = v;
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < NY4; ++q) {
+      const int e = tid + q * NT;
+      *reinterpret_cast<float4*>(&sY[buf][e / (kWidth / 4)][4 * (e % (kWidth / 4))]) = ry[q];
+    }
+#pragma unroll
+    for (int q = 0; q < NX4; ++q) {
+      const int e = tid + q * NT;
+      *reinterpret_cast<float4*>(&sX[buf][e / (K / 4)][4 * (e % (K / 4))]) = rx[q];
+    }
+  };
+
+  f32x16 acc[TNW][TKW];
+#pragma unroll
+  for (int i = 0; i < TNW; ++i)
+#pragma unroll
+    for (int j = 0; j < TKW; ++j)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) acc[i][j][v] = 0.f;
+  float bs[TNW];
+#pragma unroll
+  for (int i = 0; i < TNW; ++i) bs[i] = 0.f;
+
+  const int64_t nch = r_end > r_begin ? (r_end - r_begin + CH - 1) / CH : 0;
+  if (nch > 0) {
+    load(r_begin);
+    store(0);
   }
+  __syncthreads();
+  for (int64_t c = 0; c < nch; ++c) {
+    const int buf = (int)(c & 1);
+    if (c + 1 < nch) load(r_begin + (c + 1) * CH);
+#pragma unroll
+    for (int st = 0; st < CH / 2; ++st) {
+      float a[TNW], b[TKW];
+#pragma unroll
+      for (int i = 0; i < TNW; ++i) {
+        a[i] = sY[buf][2 * st + h][32 * (tn0 + i) + rl];
+        bs[i] += a[i];
+      }
+#pragma unroll
+      for (int j = 0; j < TKW; ++j) b[j] = sX[buf][2 * st + h][32 * (tk0 + j) + rl];
+#pragma unroll
+      for (int i = 0; i < TNW; ++i)
+#pragma unroll
+        for (int j = 0; j < TKW; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    if (c + 1 < nch) store(buf ^ 1);
+    __syncthreads();
+  }
+  float* pb = part + (int64_t)blockIdx.x * kWidth * K;
+#pragma unroll
+  for (int i = 0; i < TNW; ++i)
+#pragma unroll
+    for (int j = 0; j < TKW; ++j)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int n = dfeat(tn0 + i, v, h);
+        pb[n * K + 32 * (tk0 + j) + rl] = acc[i][j][v];
+      }
+  if (wave < NWN) {
+#pragma unroll
+    for (int i = 0; i < TNW; ++i) {
+      const float t = bs[i] + __shfl_xor(bs[i], 32, 64);
+      if (h == 0) partb[(int64_t)blockIdx.x * kWidth + 32 * (tn0 + i) + rl] = t;
+    }
+  }
+}
+
+// Deterministic two-pass sum of nblk partial slices of n floats each:
+//   pass 1: red[g][i] = sum of part[b][i] over b in group g (kReduceGroups groups),
+//   pass 2: out[r * k_real + c] = sum over g of red[g][r * k_pad + c], c < k_real.
+constexpr int kReduceGroups = 16;
+
+__global__ void k_reduce_pass1(const float* __restrict__ part, int nblk, int n,
+                               float* __restrict__ red) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = blockIdx.y;
+  if (i >= n) return;
+  const int b0 = (int)((int64_t)nblk * g / kReduceGroups);
+  const int b1 = (int)((int64_t)nblk * (g + 1) / kReduceGroups);
+  float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+  int b = b0;
+  for (; b + 4 <= b1; b += 4) {
+    s0 += part[(int64_t)b * n + i];
+    s1 += part[(int64_t)(b + 1) * n + i];
+    s2 += part[(int64_t)(b + 2) * n + i];
+    s3 += part[(int64_t)(b + 3) * n + i];
+  }
+  for (; b < b1; ++b) s0 += part[(int64_t)b * n + i];
+  red[(int64_t)g * n + i] = (s0 + s1) + (s2 + s3);
+}
+
+__global__ void k_reduce_pass2(const float* __restrict__ red, int n, int rows, int k_pad,
+                               int k_real, float* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows * k_real) return;
+  const int r = i / k_real, c = i % k_real;
+  float s = 0.f;
+#pragma unroll
+  for (int g = 0; g < kReduceGroups; ++g) s += red[(int64_t)g * n + r * k_pad + c];
+  out[i] = s;
+}
+
+// Sum nblk slices [rows, k_pad] of `part` into out [rows, k_real]; `red` holds
+// kReduceGroups slices.
+static void reduce_slices(const float* part, int nblk, int rows, int k_pad, int k_real,
+                          float* red, float* out, hipStream_t st) {
+  const int n = rows * k_pad;
+  dim3 g1((unsigned)cdiv(n, 256), kReduceGroups);
+  k_reduce_pass1<<<g1, 256, 0, st>>>(part, nblk, n, red);
+  k_reduce_pass2<<<(unsigned)cdiv(rows * k_real, 256), 256, 0, st>>>(red, n, rows, k_pad, k_real, out);
 }
 
 __global__ void k_f32_to_bf16(const float* __restrict__ x, int64_t n, __bf16* __restrict__ y) {
@@ -404,15 +529,31 @@ static int num_cus() {
 template <int KT, int NTO, int MODE, bool BF>
 static void launch_layer(const void* in, int in_stride, int64_t M, const void* wpk,
                          const float* bias, const void* mask, int mask_stride, void* out,
-                         int out_stride, const float* w3, const float* b3, float* logits,
-                         hipStream_t st) {
+                         int out_stride, hipStream_t st) {
   const int64_t tiles = (M + 31) / 32;
   int64_t blocks = (tiles + 7) / 8;
   const int64_t cap = num_cus();
   if (blocks > cap) blocks = cap;
   if (blocks < 1) blocks = 1;
   k_mlp_layer<KT, NTO, MODE, BF><<<(unsigned)blocks, 512, 0, st>>>(
-      in, in_stride, M, wpk, bias, mask, mask_stride, out, out_stride, w3, b3, logits);
+      in, in_stride, M, wpk, bias, mask, mask_stride, out, out_stride);
+}
+
+// Row blocks of the weight-gradient kernel: one per CU (each holds two row chunks in
+// LDS), at least one chunk of rows per block.
+static int wgrad_blocks(int64_t M, int kt) {
+  (void)kt;
+  int64_t b = cdiv(M, kWgradChunk);
+  const int64_t cap = num_cus();
+  if (b > cap) b = cap;
+  return (int)(b < 1 ? 1 : b);
+}
+
+static int last_blocks(int64_t M) {
+  int64_t b = cdiv(M, 64);
+  const int64_t cap = 4 * (int64_t)num_cus();
+  if (b > cap) b = cap;
+  return (int)(b < 1 ? 1 : b);
 }
 
 }  // namespace vx
@@ -443,18 +584,24 @@ int vx_mlp_layer_fwd(int bf16, int kt, int last, const void* in, int in_stride, 
   VX_CHECK_ARG(M >= 0, "mlp_layer_fwd: M < 0");
   VX_CHECK_ARG(kt == 2 || kt == 3 || kt == 6, "mlp_layer_fwd: input width must be 64, 96 or 192");
   if (M == 0) return VX_OK;
+  VX_CHECK_ARG(!last || (w3 && b3 && logits), "mlp_layer_fwd: last layer needs w3, b3, logits");
   hipStream_t st = (hipStream_t)stream;
-#define L(KT, MODE, BF) launch_layer<KT, 6, MODE, BF>(in, in_stride, M, wpk, bias, nullptr, 0, out, out_stride, w3, b3, logits, st)
+#define L(KT, BF) launch_layer<KT, 6, 0, BF>(in, in_stride, M, wpk, bias, nullptr, 0, out, out_stride, st)
   if (!bf16) {
-    if (kt == 2) { if (last) L(2, 1, false); else L(2, 0, false); }
-    else if (kt == 3) { if (last) L(3, 1, false); else L(3, 0, false); }
-    else { if (last) L(6, 1, false); else L(6, 0, false); }
+    if (kt == 2) L(2, false);
+    else if (kt == 3) L(3, false);
+    else L(6, false);
   } else {
-    if (kt == 2) { if (last) L(2, 1, true); else L(2, 0, true); }
-    else if (kt == 3) { if (last) L(3, 1, true); else L(3, 0, true); }
-    else { if (last) L(6, 1, true); else L(6, 0, true); }
+    if (kt == 2) L(2, true);
+    else if (kt == 3) L(3, true);
+    else L(6, true);
   }
 #undef L
+  if (last) {
+    const unsigned g = grid_for(M, 4, 8 * (int64_t)num_cus());
+    if (bf16) k_mlp_logits<true><<<g, 256, 0, st>>>(out, out_stride, M, w3, b3, logits);
+    else k_mlp_logits<false><<<g, 256, 0, st>>>(out, out_stride, M, w3, b3, logits);
+  }
   return launch_status("mlp_layer_fwd");
 }
 
@@ -465,7 +612,7 @@ int vx_mlp_layer_bwd(int bf16, int nto, const float* dy, int dy_stride, int64_t 
   VX_CHECK_ARG(nto == 2 || nto == 3 || nto == 6, "mlp_layer_bwd: output width must be 64, 96 or 192");
   if (M == 0) return VX_OK;
   hipStream_t st = (hipStream_t)stream;
-#define L(NTO, MODE, BF) launch_layer<6, NTO, MODE, BF>(dy, dy_stride, M, wpk_t, nullptr, mask, mask_stride, dx, dx_stride, nullptr, nullptr, nullptr, st)
+#define L(NTO, MODE, BF) launch_layer<6, NTO, MODE, BF>(dy, dy_stride, M, wpk_t, nullptr, mask, mask_stride, dx, dx_stride, st)
   if (!bf16) {
     if (nto == 6) { if (mask) L(6, 2, false); else L(6, 3, false); }
     else if (nto == 3) L(3, 3, false);
@@ -479,65 +626,54 @@ int vx_mlp_layer_bwd(int bf16, int nto, const float* dy, int dy_stride, int64_t 
   return launch_status("mlp_layer_bwd");
 }
 
-int vx_mlp_dh_last(int bf16, const float* dlogit, const void* H, int h_stride, int64_t M,
-                   const float* w3, float* dH, int dh_stride, void* stream) {
-  if (M == 0) return VX_OK;
-  hipStream_t st = (hipStream_t)stream;
-  if (bf16)
-    k_mlp_dh_last<true><<<grid_for(M * kWidth, 256), 256, 0, st>>>(dlogit, H, h_stride, M, w3, dH, dh_stride);
-  else
-    k_mlp_dh_last<false><<<grid_for(M * kWidth, 256), 256, 0, st>>>(dlogit, H, h_stride, M, w3, dH, dh_stride);
-  return launch_status("mlp_dh_last");
+int64_t vx_mlp_wgrad_part_elems(int64_t M) {
+  int64_t need = 0;
+  for (int kt : {2, 3, 6}) {
+    const int64_t e = ((int64_t)wgrad_blocks(M, kt) + kReduceGroups) * kWidth * 32 * kt;
+    if (e > need) need = e;
+  }
+  const int64_t e3 = ((int64_t)last_blocks(M) + kReduceGroups) * 3 * kWidth;
+  return need > e3 ? need : e3;
 }
 
-// Number of row blocks the weight-gradient kernels use for M rows (the partial buffers
-// hold that many [192 x K] slices).
-int vx_mlp_wgrad_blocks(int64_t M) {
-  int64_t b = (M + 255) / 256;
-  const int64_t cap = 4 * (int64_t)num_cus();
-  if (b > cap) b = cap;
-  return (int)(b < 1 ? 1 : b);
+int64_t vx_mlp_wgrad_partb_elems(int64_t M) {
+  int64_t nb = last_blocks(M);
+  for (int kt : {2, 3, 6})
+    if (wgrad_blocks(M, kt) > nb) nb = wgrad_blocks(M, kt);
+  return (nb + kReduceGroups) * kWidth;
 }
 
 int vx_mlp_wgrad(int bf16, int kt, int k_real, const float* dY, int dys, const void* X, int xs,
                  int64_t M, float* part, float* partb, float* dW, float* db, void* stream) {
   VX_CHECK_ARG(kt == 2 || kt == 3 || kt == 6, "mlp_wgrad: input width must be 64, 96 or 192");
+  VX_CHECK_ARG(k_real >= 1 && k_real <= 32 * kt, "mlp_wgrad: k_real out of range");
+  VX_CHECK_ARG(M >= 0, "mlp_wgrad: M < 0");
   hipStream_t st = (hipStream_t)stream;
-  const int nblk = vx_mlp_wgrad_blocks(M);
-  const int64_t rpb = ((M + nblk - 1) / nblk + 31) / 32 * 32;
+  const int nblk = wgrad_blocks(M, kt);
+  const int64_t rpb = cdiv(cdiv(M, nblk), kWgradChunk) * kWgradChunk;
   const int K = 32 * kt;
-  if (M > 0) {
-    if (kt == 2) {
-      if (bf16) k_mlp_wgrad<2, true><<<nblk, 256, 0, st>>>(dY, dys, X, xs, M, rpb, part, partb);
-      else k_mlp_wgrad<2, false><<<nblk, 256, 0, st>>>(dY, dys, X, xs, M, rpb, part, partb);
-    } else if (kt == 3) {
-      if (bf16) k_mlp_wgrad<3, true><<<nblk, 256, 0, st>>>(dY, dys, X, xs, M, rpb, part, partb);
-      else k_mlp_wgrad<3, false><<<nblk, 256, 0, st>>>(dY, dys, X, xs, M, rpb, part, partb);
-    } else {
-      if (bf16) k_mlp_wgrad<6, true><<<nblk, 256, 0, st>>>(dY, dys, X, xs, M, rpb, part, partb);
-      else k_mlp_wgrad<6, false><<<nblk, 256, 0, st>>>(dY, dys, X, xs, M, rpb, part, partb);
-    }
-    k_reduce_parts<<<grid_for((int64_t)kWidth * k_real, 256), 256, 0, st>>>(part, nblk, kWidth, K, k_real, dW);
-    k_reduce_parts<<<1, 256, 0, st>>>(partb, nblk, 1, kWidth, kWidth, db);
-  } else {
-    k_reduce_parts<<<grid_for((int64_t)kWidth * k_real, 256), 256, 0, st>>>(part, 0, kWidth, K, k_real, dW);
-    k_reduce_parts<<<1, 256, 0, st>>>(partb, 0, 1, kWidth, kWidth, db);
-  }
+#define L(KT, BF) k_mlp_wgrad<KT, BF><<<nblk, 64 * WgradShape<KT>::WAVES, 0, st>>>(dY, dys, X, xs, M, rpb, part, partb)
+  if (kt == 2) { if (bf16) L(2, true); else L(2, false); }
+  else if (kt == 3) { if (bf16) L(3, true); else L(3, false); }
+  else { if (bf16) L(6, true); else L(6, false); }
+#undef L
+  reduce_slices(part, nblk, kWidth, K, k_real, part + (int64_t)nblk * kWidth * K, dW, st);
+  reduce_slices(partb, nblk, 1, kWidth, kWidth, partb + (int64_t)nblk * kWidth, db, st);
   return launch_status("mlp_wgrad");
 }
 
-int vx_mlp_wgrad_last(int bf16, const float* dlogit, const void* H, int hs, int64_t M,
-                      float* part, float* partb, float* dW3, float* db3, void* stream) {
+int vx_mlp_last_bwd(int bf16, const float* dlogit, const void* H, int hs, int64_t M,
+                    const float* w3, float* dH, int dhs, float* part, float* partb, float* dW3,
+                    float* db3, void* stream) {
+  VX_CHECK_ARG(M >= 0, "mlp_last_bwd: M < 0");
   hipStream_t st = (hipStream_t)stream;
-  const int nblk = vx_mlp_wgrad_blocks(M);
-  const int64_t rpb = (M + nblk - 1) / nblk;
-  if (M > 0) {
-    if (bf16) k_mlp_wgrad_last<true><<<nblk, 192, 0, st>>>(dlogit, H, hs, M, rpb, part, partb);
-    else k_mlp_wgrad_last<false><<<nblk, 192, 0, st>>>(dlogit, H, hs, M, rpb, part, partb);
-  }
-  k_reduce_parts<<<grid_for(3 * kWidth, 256), 256, 0, st>>>(part, M > 0 ? nblk : 0, 3, kWidth, kWidth, dW3);
-  k_reduce_parts<<<1, 64, 0, st>>>(partb, M > 0 ? nblk : 0, 1, 3, 3, db3);
-  return launch_status("mlp_wgrad_last");
+  const int nblk = last_blocks(M);
+  const int64_t rpb = cdiv(M, nblk);
+  if (bf16) k_mlp_last_bwd<true><<<nblk, kWidth, 0, st>>>(dlogit, H, hs, M, rpb, w3, dH, dhs, part, partb);
+  else k_mlp_last_bwd<false><<<nblk, kWidth, 0, st>>>(dlogit, H, hs, M, rpb, w3, dH, dhs, part, partb);
+  reduce_slices(part, nblk, 3, kWidth, kWidth, part + (int64_t)nblk * 3 * kWidth, dW3, st);
+  reduce_slices(partb, nblk, 1, 3, 3, partb + (int64_t)nblk * 3, db3, st);
+  return launch_status("mlp_last_bwd");
 }
 
 int vx_f32_to_bf16(const float* x, int64_t n, void* y, void* stream) {

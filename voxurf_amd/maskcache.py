@@ -41,8 +41,9 @@ class MaskCache(nn.Module):
         shape = xyz.shape[:-1]
         pts = xyz.reshape(-1, 3).to(self.density.dtype).contiguous()
         out = torch.empty(pts.shape[0], dtype=torch.bool, device=pts.device)
+        geom = grid_geom(self.density)  # host array: must outlive the call
         call("vx_maskcache_alpha", dtype_code(self.density), ptr(self.density),
-             grid_geom(self.density).data_ptr(), ptr(pts), ptr(self.xyz_min), ptr(self.xyz_max),
+             geom.data_ptr(), ptr(pts), ptr(self.xyz_min), ptr(self.xyz_max),
              self.act_shift, self.voxel_size_ratio, float(self.mask_cache_thres),
              int(self.nearest), pts.shape[0], ptr(out), stream_of(pts))
         return out.reshape(shape)

@@ -75,18 +75,15 @@ class _MlpFn(torch.autograd.Function):
         g = glogits.contiguous().float()
         nlin = len(Ws)
         grads = [None] * (2 * nlin)
-        nblk = lib().vx_mlp_wgrad_blocks(M)
-        part = torch.empty(nblk * WIDTH * WIDTH, dtype=torch.float32, device=dev)
-        partb = torch.empty(nblk * WIDTH, dtype=torch.float32, device=dev)
-        # output layer (K = 3)
+        part = torch.empty(lib().vx_mlp_wgrad_part_elems(M), dtype=torch.float32, device=dev)
+        partb = torch.empty(lib().vx_mlp_wgrad_partb_elems(M), dtype=torch.float32, device=dev)
+        # output layer (K = 3) and the gradient of the last hidden activations
         dW = torch.empty_like(Ws[-1])
         db = torch.empty(3, dtype=torch.float32, device=dev)
-        call("vx_mlp_wgrad_last", int(bf16), ptr(g), ptr(hs[-1]), WIDTH, M, ptr(part), ptr(partb),
-             ptr(dW), ptr(db), st)
-        grads[2 * (nlin - 1)], grads[2 * (nlin - 1) + 1] = dW, db
         dH = torch.empty((M, WIDTH), dtype=torch.float32, device=dev)
-        call("vx_mlp_dh_last", int(bf16), ptr(g), ptr(hs[-1]), WIDTH, M, ptr(Ws[-1]), ptr(dH),
-             WIDTH, st)
+        call("vx_mlp_last_bwd", int(bf16), ptr(g), ptr(hs[-1]), WIDTH, M, ptr(Ws[-1]), ptr(dH),
+             WIDTH, ptr(part), ptr(partb), ptr(dW), ptr(db), st)
+        grads[2 * (nlin - 1)], grads[2 * (nlin - 1) + 1] = dW, db
         dX = None
         for li in range(nlin - 2, -1, -1):
             h_prev = hs[li - 1] if li > 0 else X
