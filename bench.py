@@ -33,6 +33,8 @@ def parse():
     ap.add_argument("--mlp", choices=["fp32", "bf16"], default="fp32")
     ap.add_argument("--rays", type=int, default=8192)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serial-adam", action="store_true",
+                    help="run the k0 Adam pass in order on the main stream (per-kernel timing)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--mode", choices=["train", "coarse", "recompute", "precision", "serve",
                                        "kernels"],
@@ -113,6 +115,7 @@ def main():
     mlp = scene.mlp_params(cfg.mlp_in, cfg.mlp_width, cfg.mlp_depth)
     if args.path == "fused":
         trainer = FusedFineTrainer(cfg, sdf, k0, mlp, device=dev, mlp_dtype=mlp_dtype)
+        trainer.overlap_adam = not args.serial_adam
     else:
         model = VoxurfFine(cfg, sdf, k0, mlp, mlp_dtype=mlp_dtype).to(dev)
         trainer = Trainer(model, cfg)

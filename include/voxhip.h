@@ -229,13 +229,15 @@ int vx_segment_gather(int dtype, const void* gout, const int64_t* index, int64_t
  * grid_sampler with sample_grad, neus_alpha_from_sdf_scatter, the two fast_color_thres
  * compactions and Alphas2Weights) into one pass per ray, and their adjoints
  * (Alphas2Weights.backward -> render_utils_kernel.cu:654-677, the NeuS and grid_sample
- * adjoints that autograd runs) into a per-ray recomputing backward.  Host-side arrays:
- * sdf_geom/k0_geom as for the grid ops; box[6] = {xyz_min, xyz_max};
+ * adjoints that autograd runs) into a per-ray backward over the stored walk range.
+ * Host-side arrays: sdf_geom/k0_geom as for the grid ops; box[6] = {xyz_min, xyz_max};
  * params[6] = {stepdist, voxel_size, inv_s, fast_color_thres, near, far}.
- * Per-ray survivor slots hold `cap` entries of two float4 each (s4: x, y, z, weight;
- * g4: gradient xyz, step).  mc (may be NULL): the MaskCache density grid (device, one
- * channel, geometry mc_geom) with mc_params[10] = {act_shift, voxel_size_ratio, thres,
- * nearest, xyz_min[3], xyz_max[3]}; samples it rejects are dropped like out-of-box ones
+ * Per-ray slots hold `cap` walk-range samples (alpha > thres, up to the terminating
+ * one) of two float4 each: s4 = (x, y, z, transmittance before the sample),
+ * g4 = (SDF gradient xyz, SDF value); stats[3 r + 1] is the number of slots used.
+ * mc (may be NULL): the MaskCache density grid (device, one channel, geometry mc_geom)
+ * with mc_params[10] = {act_shift, voxel_size_ratio, thres, nearest, xyz_min[3],
+ * xyz_max[3]}; samples it rejects are dropped like out-of-box ones
  * (lib/voxurf_fine.py:631-636). */
 int vx_fused_fine_fwd(const float* sdf, const int64_t* sdf_geom, const float* box,
                       const float* params, const float* mc, const int64_t* mc_geom,
@@ -243,14 +245,16 @@ int vx_fused_fine_fwd(const float* sdf, const int64_t* sdf_geom, const float* bo
                       const float* viewdirs, int64_t n_rays, int cap, void* s4, void* g4,
                       int64_t* cnt, float* last, int* stats, int count_all, int* overflow,
                       void* stream);
-/* Compacts survivors to rows off_incl[r]-cnt[r] .. off_incl[r]-1 and builds the rgbnet
- * input rows X [M, xstride] = [k0 | normal | view PE | 0] (lib/voxurf_fine.py:678-716);
- * P4/G4 [M] float4 copies of s4/g4, rid [M] int32 ray ids. */
+/* Keeps the survivors (weight > thres) of each ray's slots at rows
+ * off_incl[r]-cnt[r] .. off_incl[r]-1 and builds the rgbnet input rows
+ * X [M, xstride] = [k0 | normal | view PE | 0] (lib/voxurf_fine.py:678-716; xstride a
+ * multiple of 4, at most 64); P4 [M] = (x, y, z, weight), G4 [M] = (gradient, sdf),
+ * rid [M] int32 ray ids. */
 int vx_fused_fine_gather(const float* k0, const int64_t* k0_geom, const int64_t* sdf_geom,
                          const float* box, const float* params, const float* viewdirs,
                          int64_t n_rays, int cap, int n_freq, const void* s4, const void* g4,
-                         const int64_t* cnt, const int64_t* off_incl, int xstride, float* X,
-                         void* P4, void* G4, int* rid, void* stream);
+                         const int* stats, const int64_t* cnt, const int64_t* off_incl,
+                         int xstride, float* X, void* P4, void* G4, int* rid, void* stream);
 /* segment_coo compositing (lib/voxurf_fine.py:753-763), MSE loss per ray (run.py:604)
  * and the colour-side adjoints: dlogit [M,3], gw [M] (d loss / d weight), glast [N],
  * S [N] (sum of gw*w per ray), loss_r [N] (sum of squared errors).  With target NULL
@@ -266,12 +270,11 @@ int vx_fused_fine_feat_bwd(const int64_t* k0_geom, const int64_t* sdf_geom, cons
                            const float* params, int64_t m, const float* dX, int dxs,
                            const void* P4, const void* G4, float* k0_grad, float* dgf,
                            uint32_t* k0_pages, void* stream);
-/* Per-ray recomputing backward of the SDF side: sdf_grad += adjoint of (7-point SDF
- * lookup -> NeuS alpha -> transmittance walk) for every sample of the walk range. */
+/* Backward of the SDF side over each ray's stored walk range: sdf_grad += adjoint of
+ * (7-point SDF lookup -> NeuS alpha -> transmittance walk). */
 int vx_fused_fine_bwd(const float* sdf, const int64_t* sdf_geom, const float* box,
-                      const float* params, const float* mc, const int64_t* mc_geom,
-                      const float* mc_params, const float* rays_o, const float* rays_d,
-                      const float* viewdirs, int64_t n_rays, const int64_t* cnt,
+                      const float* params, const float* viewdirs, int64_t n_rays, int cap,
+                      const void* s4, const void* g4, const int* stats, const int64_t* cnt,
                       const int64_t* off_incl, const float* last, const float* gw,
                       const float* glast, const float* S, const float* dgf, float* sdf_grad,
                       void* stream);

@@ -6,9 +6,12 @@
 // walk the transmittance per ray, compact weight > thres, then features and colour.
 // Here one wavefront owns one ray and does all of that in registers, 64 consecutive
 // steps per iteration, and stops once the transmittance walk has terminated (every
-// later sample has weight 0 and no gradient in the reference).  Only the surviving
-// samples (weight > thres) leave the kernel, in per-ray slots.  The backward recomputes
-// each ray the same way instead of storing every sample.
+// later sample has weight 0 and no gradient in the reference).  Only the samples of the
+// walk range (alpha > thres, up to and including the terminating one: the only samples
+// with a gradient) leave the kernel, in per-ray slots of two float4 (position and the
+// transmittance before the sample; SDF gradient and SDF value).  The gather and the
+// backward recompute alpha and weight from those (the same arithmetic, so the same
+// bits) instead of walking the ray again.
 //
 // The arithmetic of every sample is the drop-in ops' own (voxel.h), and the
 // transmittance walk is the reference's sequential float recurrence, so the survivor
@@ -103,6 +106,44 @@ __device__ __forceinline__ void run_chunk(const FusedGeom& G, const float* start
   cs.surv = cs.inrange && cs.w > G.thres;
 }
 
+// Alpha and weight of a stored walk-range sample, recomputed with the forward's own
+// arithmetic (bit-identical to the walk's values).
+struct SlotSample {
+  float q[3], grad[3], sdf, myT, alpha, w;
+  NeusFwd<float> nf;
+};
+
+__device__ __forceinline__ void load_slot(const FusedGeom& G, const float* vd, float4 a, float4 b,
+                                          SlotSample& ss) {
+#pragma clang fp contract(off)
+  ss.q[0] = a.x;
+  ss.q[1] = a.y;
+  ss.q[2] = a.z;
+  ss.myT = a.w;
+  ss.grad[0] = b.x;
+  ss.grad[1] = b.y;
+  ss.grad[2] = b.z;
+  ss.sdf = b.w;
+  ss.nf = neus_fwd<float>(vd, ss.grad, ss.sdf, G.stepdist, G.inv_s);
+  ss.alpha = clip01(ss.nf.a);
+  ss.w = ss.myT * ss.alpha;
+}
+
+// Position of the k-th set bit of m (k < popcount(m)).
+__device__ __forceinline__ int kth_set_bit(uint64_t m, int k) {
+  int pos = 0;
+#pragma unroll
+  for (int width = 32; width >= 1; width >>= 1) {
+    const uint64_t low = (m >> pos) & ((1ull << width) - 1ull);
+    const int c = __popcll(low);
+    if (k >= c) {
+      k -= c;
+      pos += width;
+    }
+  }
+  return pos;
+}
+
 __device__ __forceinline__ void ray_setup(const FusedGeom& G, const float* o, const float* d,
                                           float* start, float* dir, int64_t& n) {
   float tmin, tmax;
@@ -112,8 +153,9 @@ __device__ __forceinline__ void ray_setup(const FusedGeom& G, const float* o, co
 }
 
 // Forward.  Outputs per ray: cnt (survivors), last (alphainv_last), stats
-// {m0 processed, m1 in the walk range, m0 of the whole ray (if count_all)};
-// per survivor k of ray r at slot r*cap + k: s4 = (x, y, z, w), g4 = (gx, gy, gz, step).
+// {m0 processed, m1 in the walk range (= slots used), m0 of the whole ray (if count_all)};
+// per walk-range sample k of ray r at slot r*cap + k: s4 = (x, y, z, T before the
+// sample), g4 = (gx, gy, gz, sdf).
 __global__ void __launch_bounds__(256) k_fused_fwd(
     FusedGeom G, const float* __restrict__ rays_o, const float* __restrict__ rays_d,
     const float* __restrict__ viewdirs, int64_t n_rays, int cap, float4* __restrict__ s4,
@@ -135,17 +177,17 @@ __global__ void __launch_bounds__(256) k_fused_fwd(
       ChunkState cs;
       run_chunk(G, start, dir, vd, n, base, lane, T, stopped, cs);
       m0 += __popcll(__ballot(cs.inb));
-      m1 += __popcll(__ballot(cs.inrange));
-      const uint64_t sm = __ballot(cs.surv);
-      if (cs.surv) {
-        const int k = cnt + __popcll(sm & lanemask_lt(lane));
+      const uint64_t im = __ballot(cs.inrange);
+      if (cs.inrange) {
+        const int k = m1 + __popcll(im & lanemask_lt(lane));
         if (k < cap) {
           const int64_t slot = r * cap + k;
-          s4[slot] = make_float4(cs.q[0], cs.q[1], cs.q[2], cs.w);
-          g4[slot] = make_float4(cs.grad[0], cs.grad[1], cs.grad[2], (float)(base + lane));
+          s4[slot] = make_float4(cs.q[0], cs.q[1], cs.q[2], cs.myT);
+          g4[slot] = make_float4(cs.grad[0], cs.grad[1], cs.grad[2], cs.sdf);
         }
       }
-      cnt += __popcll(sm);
+      m1 += __popcll(im);
+      cnt += __popcll(__ballot(cs.surv));
     }
     int m0_all = m0;
     if (count_all) {
@@ -159,9 +201,9 @@ __global__ void __launch_bounds__(256) k_fused_fwd(
       }
     }
     if (lane == 0) {
-      if (cnt > cap) {
+      if (m1 > cap) {
         atomicOr(overflow, 1);
-        cnt = cap;
+        m1 = cap;
       }
       cnt_out[r] = cnt;
       last_out[r] = T;
@@ -172,16 +214,18 @@ __global__ void __launch_bounds__(256) k_fused_fwd(
   }
 }
 
-// Compaction + rgbnet input.  One wavefront per ray copies its survivors to
-// off[r] .. off[r] + cnt[r] and builds X = [k0 (C) | normal (3) | view PE (3 + 6 nf) | 0 pad].
+// Compaction + rgbnet input.  One wavefront per ray takes its walk-range slots 64 at a
+// time, keeps the survivors (w > thres) at rows off[r] .. off[r] + cnt[r] and builds
+// X = [k0 (C) | normal (3) | view PE (3 + 6 nf) | 0 pad], four rows per pass with
+// sixteen lanes per row and four columns per lane.
 // normal = grad / (|grad| + 1e-5) (lib/voxurf_fine.py:766 with the 1e-5 of :810);
 // view PE = [v, sin(v_i f_j), cos(v_i f_j)] with f_j = 2^j (lib/voxurf_fine.py:694-695).
 __global__ void __launch_bounds__(256) k_fused_gather(
     FusedGeom G, const float* __restrict__ viewdirs, int64_t n_rays, int cap, int n_freq,
-    const float4* __restrict__ s4, const float4* __restrict__ g4, const int64_t* __restrict__ cnt,
-    const int64_t* __restrict__ off_incl, int xstride, float* __restrict__ X,
-    float4* __restrict__ P4, float4* __restrict__ G4, int* __restrict__ rid) {
-  const int lane = threadIdx.x & 63;
+    const float4* __restrict__ s4, const float4* __restrict__ g4, const int* __restrict__ stats,
+    const int64_t* __restrict__ cnt, const int64_t* __restrict__ off_incl, int xstride,
+    float* __restrict__ X, float4* __restrict__ P4, float4* __restrict__ G4, int* __restrict__ rid) {
+  const int lane = threadIdx.x & 63, grp = lane >> 4, cl = lane & 15;
   const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
   const int C = G.gk.C;
   const int npe = 3 + 6 * n_freq;
@@ -189,49 +233,83 @@ __global__ void __launch_bounds__(256) k_fused_gather(
        r += waves) {
     const int64_t c = cnt[r];
     if (c == 0) continue;
+    const int c1 = stats[3 * r + 1];
     const int64_t o = off_incl[r] - c;
-    // view PE of this ray, lane e holds element e
-    float pe = 0.f;
-    {
-      const float* v = viewdirs + 3 * r;
-      if (lane < 3) {
-        pe = v[lane];
-      } else if (lane < npe) {
-        const int e = lane - 3;
+    const float* vd = viewdirs + 3 * r;
+    // view PE values of this lane's four columns (zero outside the PE block)
+    float pev[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int idx = 4 * cl + e - C - 3;
+      float v = 0.f;
+      if (idx >= 0 && idx < 3) {
+        v = vd[idx];
+      } else if (idx >= 3 && idx < npe) {
         const int half = 3 * n_freq;
-        const int idx = e % half;  // (component, freq) in component-major order
-        const int comp = idx / n_freq, f = idx % n_freq;
-        const float arg = v[comp] * (float)(1 << f);
-        pe = (e < half) ? sinf(arg) : cosf(arg);
+        const int q = (idx - 3) % half;  // (component, freq) in component-major order
+        const int comp = q / n_freq, f = q % n_freq;
+        const float arg = vd[comp] * (float)(1 << f);
+        v = (idx - 3 < half) ? sinf(arg) : cosf(arg);
       }
+      pev[e] = v;
     }
-    for (int64_t k = 0; k < c; ++k) {
-      // one survivor per iteration; lanes cooperate on the feature row
-      const float4 a = s4[r * cap + k];
-      const float4 b = g4[r * cap + k];
-      const int64_t j = o + k;
-      const float q[3] = {a.x, a.y, a.z};
-      Tri<float> t;
-      t.init(world_to_index(q[0], G.mn[0], G.mx[0] - G.mn[0], G.gk.X),
-             world_to_index(q[1], G.mn[1], G.mx[1] - G.mn[1], G.gk.Y),
-             world_to_index(q[2], G.mn[2], G.mx[2] - G.mn[2], G.gk.Z));
-      float val = 0.f;
-      if (lane < C) {
-        val = tri_gather(G.k0, G.gk, t, lane);
-      } else if (lane < C + 3) {
-        const float gx = b.x, gy = b.y, gz = b.z;
-        const float nrm = sqrtf(gx * gx + gy * gy + gz * gz) + 1e-5f;
-        const float comp = (lane == C) ? gx : (lane == C + 1 ? gy : gz);
-        val = comp / nrm;
+    int64_t ks = 0;
+    for (int base = 0; base < c1; base += 64) {
+      const int k = base + lane;
+      float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = a;
+      bool surv = false;
+      float w = 0.f;
+      if (k < c1) {
+        a = s4[r * cap + k];
+        b = g4[r * cap + k];
+        SlotSample ss;
+        load_slot(G, vd, a, b, ss);
+        w = ss.w;
+        surv = ss.w > G.thres;
       }
-      const float pv = __shfl(pe, (lane - C - 3) & 63, 64);
-      if (lane >= C + 3 && lane < C + 3 + npe) val = pv;
-      if (lane < xstride) X[j * xstride + lane] = val;
-      if (lane == 0) {
-        P4[j] = a;
+      const uint64_t sm = __ballot(surv);
+      // rows past the forward's survivor count are never written (they cannot occur:
+      // the recomputed weights are the forward's)
+      const int64_t room = c - ks;
+      const int rank = __popcll(sm & lanemask_lt(lane));
+      if (surv && rank < room) {
+        const int64_t j = o + ks + rank;
+        P4[j] = make_float4(a.x, a.y, a.z, w);
         G4[j] = b;
         rid[j] = (int)r;
       }
+      const int ns = (int)(__popcll(sm) < room ? __popcll(sm) : room);
+      for (int i = 0; i < ns; i += 4) {
+        const int kk = i + grp;
+        const bool on = kk < ns;
+        const int src = kth_set_bit(sm, on ? kk : 0);
+        const float qx = __shfl(a.x, src, 64), qy = __shfl(a.y, src, 64), qz = __shfl(a.z, src, 64);
+        const float gx = __shfl(b.x, src, 64), gy = __shfl(b.y, src, 64), gz = __shfl(b.z, src, 64);
+        if (on && 4 * cl < xstride) {
+          Tri<float> t;
+          t.init(world_to_index(qx, G.mn[0], G.mx[0] - G.mn[0], G.gk.X),
+                 world_to_index(qy, G.mn[1], G.mx[1] - G.mn[1], G.gk.Y),
+                 world_to_index(qz, G.mn[2], G.mx[2] - G.mn[2], G.gk.Z));
+          const float nrm = sqrtf(gx * gx + gy * gy + gz * gz) + 1e-5f;
+          float v[4];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int col = 4 * cl + e;
+            float x = pev[e];
+            if (col < C) {
+              x = tri_gather(G.k0, G.gk, t, col);
+            } else if (col < C + 3) {
+              const float comp = (col == C) ? gx : (col == C + 1 ? gy : gz);
+              x = comp / nrm;
+            }
+            v[e] = x;
+          }
+          *reinterpret_cast<float4*>(X + (o + ks + kk) * xstride + 4 * cl) =
+              make_float4(v[0], v[1], v[2], v[3]);
+        }
+      }
+      ks += ns;
+      if (ks >= c) break;
     }
   }
 }
@@ -348,38 +426,43 @@ __global__ void __launch_bounds__(256) k_fused_feat_bwd(
   }
 }
 
-// Backward, one wavefront per ray, recomputing the forward walk:
+// Backward, one wavefront per ray over its stored walk-range samples:
 //   dalpha_i = gw_i T_i - back_i / (1 - alpha_i + 1e-10)   on the walk range,
 //   back_i = glast * last + S_r - sum_{survivors j <= i} gw_j w_j
 // (render_utils_kernel.cu:654-677 with the suffix sum written as total - prefix), then
 // the NeuS adjoint and the 7-point stencil scatter into the SDF gradient.
 __global__ void __launch_bounds__(256) k_fused_bwd(
-    FusedGeom G, const float* __restrict__ rays_o, const float* __restrict__ rays_d,
-    const float* __restrict__ viewdirs, int64_t n_rays, const int64_t* __restrict__ cnt,
-    const int64_t* __restrict__ off_incl, const float* __restrict__ last,
-    const float* __restrict__ gw, const float* __restrict__ glast, const float* __restrict__ S,
-    const float* __restrict__ dgf, float* __restrict__ sdf_grad) {
+    FusedGeom G, const float* __restrict__ viewdirs, int64_t n_rays, int cap,
+    const float4* __restrict__ s4, const float4* __restrict__ g4, const int* __restrict__ stats,
+    const int64_t* __restrict__ cnt, const int64_t* __restrict__ off_incl,
+    const float* __restrict__ last, const float* __restrict__ gw, const float* __restrict__ glast,
+    const float* __restrict__ S, const float* __restrict__ dgf, float* __restrict__ sdf_grad) {
   const int lane = threadIdx.x & 63;
   const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
   for (int64_t r = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6); r < n_rays;
        r += waves) {
-    float start[3], dir[3];
-    int64_t n;
-    ray_setup(G, rays_o + 3 * r, rays_d + 3 * r, start, dir, n);
     const float* vd = viewdirs + 3 * r;
+    const int c1 = stats[3 * r + 1];
     const int64_t c_r = cnt[r];
     const int64_t o = off_incl[r] - c_r;
     const float base_back = glast[r] * last[r] + S[r];
-    float T = 1.f, prefix = 0.f;
-    bool stopped = false;
+    float prefix = 0.f;
     int k_surv = 0;
-    for (int64_t base = 0; base < n && !stopped; base += 64) {
-      ChunkState cs;
-      run_chunk(G, start, dir, vd, n, base, lane, T, stopped, cs);
-      const uint64_t sm = __ballot(cs.surv);
+    for (int base = 0; base < c1; base += 64) {
+      const int k = base + lane;
+      const bool valid = k < c1;
+      SlotSample ss;
+      bool surv = false;
+      if (valid) {
+        load_slot(G, vd, s4[r * cap + k], g4[r * cap + k], ss);
+        surv = ss.w > G.thres;
+      } else {
+        ss.w = 0.f;
+      }
+      const uint64_t sm = __ballot(surv);
       float gwl = 0.f, dg[3] = {0.f, 0.f, 0.f};
       const int64_t kk = k_surv + __popcll(sm & lanemask_lt(lane));
-      if (cs.surv && kk < c_r) {
+      if (surv && kk < c_r) {
         const int64_t j = o + kk;
         gwl = gw[j];
         dg[0] = dgf[3 * j];
@@ -388,7 +471,7 @@ __global__ void __launch_bounds__(256) k_fused_bwd(
       }
       k_surv += __popcll(sm);
       // inclusive prefix of gw*w over survivors within the chunk
-      const float c = gwl * cs.w;
+      const float c = gwl * ss.w;
       float incl = c;
 #pragma unroll
       for (int off = 1; off < 64; off <<= 1) {
@@ -397,15 +480,15 @@ __global__ void __launch_bounds__(256) k_fused_bwd(
       }
       const float back = base_back - (prefix + incl);
       prefix += __shfl(incl, 63, 64);
-      if (cs.inrange) {
-        const double den = (double)(1.f - cs.alpha) + 1e-10;
-        const float da = (float)((double)(gwl * cs.myT) - (double)back / den);
+      if (valid) {
+        const double den = (double)(1.f - ss.alpha) + 1e-10;
+        const float da = (float)((double)(gwl * ss.myT) - (double)back / den);
         float gsdf, gg[3];
-        neus_bwd<float>(vd, cs.nf, G.stepdist, G.inv_s, da, gsdf, gg);
+        neus_bwd<float>(vd, ss.nf, G.stepdist, G.inv_s, da, gsdf, gg);
         gg[0] += dg[0];
         gg[1] += dg[1];
         gg[2] += dg[2];
-        sdf7_scatter<float>(sdf_grad, G.gs, cs.q, G.mn, G.mx, G.voxel_size, gsdf, gg, nullptr);
+        sdf7_scatter<float>(sdf_grad, G.gs, ss.q, G.mn, G.mx, G.voxel_size, gsdf, gg, nullptr);
       }
     }
   }
@@ -505,16 +588,17 @@ int vx_fused_fine_fwd(const float* sdf, const int64_t* sdf_geom, const float* bo
 int vx_fused_fine_gather(const float* k0, const int64_t* k0_geom, const int64_t* sdf_geom,
                          const float* box, const float* params, const float* viewdirs,
                          int64_t n_rays, int cap, int n_freq, const void* s4, const void* g4,
-                         const int64_t* cnt, const int64_t* off_incl, int xstride, float* X,
-                         void* P4, void* G4, int* rid, void* stream) {
+                         const int* stats, const int64_t* cnt, const int64_t* off_incl,
+                         int xstride, float* X, void* P4, void* G4, int* rid, void* stream) {
   FusedGeom G;
   if (int st = fused_geom(G, nullptr, sdf_geom, k0, k0_geom, box, params)) return st;
-  VX_CHECK_ARG(G.gk.C + 3 + 3 + 6 * n_freq <= xstride && xstride <= 64,
-               "fused_fine_gather: feature row does not fit xstride");
+  VX_CHECK_ARG(G.gk.C + 3 + 3 + 6 * n_freq <= xstride && xstride <= 64 && xstride % 4 == 0,
+               "fused_fine_gather: feature row does not fit xstride (a multiple of 4, <= 64)");
+  VX_CHECK_ARG(cap >= 1, "fused_fine_gather: cap < 1");
   if (n_rays == 0) return VX_OK;
   k_fused_gather<<<grid_for(n_rays, 4, 1 << 16), 256, 0, (hipStream_t)stream>>>(
-      G, viewdirs, n_rays, cap, n_freq, (const float4*)s4, (const float4*)g4, cnt, off_incl,
-      xstride, X, (float4*)P4, (float4*)G4, rid);
+      G, viewdirs, n_rays, cap, n_freq, (const float4*)s4, (const float4*)g4, stats, cnt,
+      off_incl, xstride, X, (float4*)P4, (float4*)G4, rid);
   return launch_status("fused_fine_gather");
 }
 
@@ -544,18 +628,18 @@ int vx_fused_fine_feat_bwd(const int64_t* k0_geom, const int64_t* sdf_geom, cons
 }
 
 int vx_fused_fine_bwd(const float* sdf, const int64_t* sdf_geom, const float* box,
-                      const float* params, const float* mc, const int64_t* mc_geom,
-                      const float* mc_params, const float* rays_o, const float* rays_d,
-                      const float* viewdirs, int64_t n_rays, const int64_t* cnt,
+                      const float* params, const float* viewdirs, int64_t n_rays, int cap,
+                      const void* s4, const void* g4, const int* stats, const int64_t* cnt,
                       const int64_t* off_incl, const float* last, const float* gw,
                       const float* glast, const float* S, const float* dgf, float* sdf_grad,
                       void* stream) {
   FusedGeom G;
   if (int st = fused_geom(G, sdf, sdf_geom, nullptr, nullptr, box, params)) return st;
-  if (int st = fused_maskcache(G, mc, mc_geom, mc_params)) return st;
+  VX_CHECK_ARG(cap >= 1, "fused_fine_bwd: cap < 1");
   if (n_rays == 0) return VX_OK;
   k_fused_bwd<<<grid_for(n_rays, 4, 1 << 16), 256, 0, (hipStream_t)stream>>>(
-      G, rays_o, rays_d, viewdirs, n_rays, cnt, off_incl, last, gw, glast, S, dgf, sdf_grad);
+      G, viewdirs, n_rays, cap, (const float4*)s4, (const float4*)g4, stats, cnt, off_incl, last,
+      gw, glast, S, dgf, sdf_grad);
   return launch_status("fused_fine_bwd");
 }
 

@@ -3,17 +3,18 @@
 One step (run.py:600-659 for the fine model, lib/voxurf_fine.py:620-802):
 
   1. vx_fused_fine_fwd     one wavefront per ray: sampling, SDF + 6-neighbour gradient,
-                           NeuS alpha, the transmittance walk; survivors (w > thres)
-                           into per-ray slots
+                           NeuS alpha, the transmittance walk; the walk-range samples
+                           (the only ones with a gradient) into per-ray slots
   2. scan of survivor counts -> row offsets; the total is read back once (the
                            reference reads sample totals back several times per step)
-  3. vx_fused_fine_gather  compaction + rgbnet input rows [k0 | normal | view PE]
+  3. vx_fused_fine_gather  survivors (w > thres) + rgbnet input rows [k0 | normal | view PE]
   4. rgbnet forward        (voxurf_amd.mlp_hip)
   5. vx_fused_fine_composite  compositing, MSE loss, colour-side adjoints
   6. rgbnet backward
   7. vx_fused_fine_feat_bwd   k0 scatter, normal adjoint
-  8. vx_fused_fine_bwd        per-ray recomputing backward of the SDF side
-  9. Adam (utils.Adam semantics) over sdf, k0 and rgbnet, clearing the grid gradients
+  8. vx_fused_fine_bwd        per-ray backward of the SDF side over the stored walk range
+  9. Adam (utils.Adam semantics) over sdf, k0 and rgbnet, clearing the grid gradients;
+     the k0 pass runs on a second stream, concurrently with step 8
 
 Grids: sdf [1,1,R,R,R] contiguous; k0 [1,C,R,R,R] channels-last.  Gradient buffers
 persist across steps (cleared by the Adam pass instead of reallocated).
@@ -28,7 +29,9 @@ from .mlp import RgbNet
 from .optim import Adam
 from .total_variation_cuda import grid_geom
 
-XSTRIDE = 48
+# rgbnet input row width: [k0 (12) | normal (3) | view PE (27)] = 42 features, zero
+# padded to the MFMA tile width so the rows are consumed in place
+XSTRIDE = 64
 
 
 def maskcache_args(mc):
@@ -76,6 +79,9 @@ class FusedFineTrainer:
         self.last_profile = {}
         self.last_counts = {}
         self.count_all = True
+        self.side = torch.cuda.Stream(self.dev)
+        # False: the k0 Adam pass runs in order on the main stream (for per-kernel timing)
+        self.overlap_adam = True
 
     mask_cache = None
 
@@ -139,8 +145,8 @@ class FusedFineTrainer:
         call("vx_fused_fine_gather", ptr(self.k0), self.k0_geom.data_ptr(),
              self.sdf_geom.data_ptr(), self.box.data_ptr(), self.params.data_ptr(),
              ptr(viewdirs), n, self.cap, self.cfg.view_pe, ptr(self.s4), ptr(self.g4),
-             ptr(self.cnt), ptr(self.off), XSTRIDE, ptr(self.X), ptr(self.P4), ptr(self.G4),
-             ptr(self.rid), st)
+             ptr(self.stats), ptr(self.cnt), ptr(self.off), XSTRIDE, ptr(self.X), ptr(self.P4),
+             ptr(self.G4), ptr(self.rid), st)
         return m
 
     def step(self, rays_o, rays_d, viewdirs, target, profile=False):
@@ -149,8 +155,8 @@ class FusedFineTrainer:
         m = self.forward(rays_o, rays_d, viewdirs)
         st = stream_of(self.sdf)
         din = cfg.mlp_in
-        xin = self.X[:m, :din].detach().requires_grad_(True)
-        logits = self.rgbnet(xin)
+        xin = self.X[:m].detach().requires_grad_(True)
+        logits = self.rgbnet.forward_padded(xin)
         lg = logits.detach().contiguous()
         dlogit = torch.empty((max(m, 1), 3), dtype=torch.float32, device=self.dev)
         gw = torch.empty(max(m, 1), dtype=torch.float32, device=self.dev)
@@ -169,17 +175,33 @@ class FusedFineTrainer:
         call("vx_fused_fine_feat_bwd", self.k0_geom.data_ptr(), self.sdf_geom.data_ptr(),
              self.box.data_ptr(), self.params.data_ptr(), m, ptr(dX), dX.stride(0), ptr(self.P4),
              ptr(self.G4), ptr(self.k0_grad), ptr(dgf), ptr(self.k0_pages), st)
-        call("vx_fused_fine_bwd", ptr(self.sdf), self.sdf_geom.data_ptr(), self.box.data_ptr(),
-             self.params.data_ptr(), *self._mc_args(), ptr(rays_o), ptr(rays_d), ptr(viewdirs), n, ptr(self.cnt),
-             ptr(self.off), ptr(self.last), ptr(gw), ptr(self.glast), ptr(self.S), ptr(dgf),
-             ptr(self.sdf_grad), st)
+        # The k0 Adam pass (HBM-bound, the largest kernel of the step) depends only on
+        # the feature scatter above: it runs on a second stream, overlapping the SDF-side
+        # backward (L2/atomic-bound) and the remaining optimiser passes.
+        main = torch.cuda.current_stream(self.dev)
         if profile:
-            self.grads_snapshot = dict(sdf=self.sdf_grad.clone(), k0=self.k0_grad.clone(),
+            # snapshot before any optimiser pass clears a gradient
+            k0_grad_snapshot = self.k0_grad.clone()
+        k0_stream = self.side if self.overlap_adam else main
+        grads_ready = torch.cuda.Event()
+        grads_ready.record(main)
+        with torch.cuda.stream(k0_stream):
+            k0_stream.wait_event(grads_ready)
+            ev = self._adam("k0", self.k0, self.k0_grad, profile=profile, pages=self.k0_pages)
+            k0_done = torch.cuda.Event()
+            k0_done.record(k0_stream)
+        call("vx_fused_fine_bwd", ptr(self.sdf), self.sdf_geom.data_ptr(), self.box.data_ptr(),
+             self.params.data_ptr(), ptr(viewdirs), n, self.cap, ptr(self.s4), ptr(self.g4),
+             ptr(self.stats), ptr(self.cnt), ptr(self.off), ptr(self.last), ptr(gw),
+             ptr(self.glast), ptr(self.S), ptr(dgf), ptr(self.sdf_grad), st)
+        if profile:
+            self.grads_snapshot = dict(sdf=self.sdf_grad.clone(), k0=k0_grad_snapshot,
                                        mlp=[p.grad.clone() for p in self.rgbnet.parameters()])
         self._adam("sdf", self.sdf, self.sdf_grad)
-        ev = self._adam("k0", self.k0, self.k0_grad, profile=profile, pages=self.k0_pages)
         self.mlp_opt.step()
         call("vx_sum_f32", ptr(self.loss_r), n, ptr(self.loss), st)
+        # the next step's gather reads k0
+        main.wait_event(k0_done)
         if profile:
             torch.cuda.synchronize()
             self.last_profile = {"adam_k0_ms": ev[0].elapsed_time(ev[1])}

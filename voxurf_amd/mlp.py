@@ -40,3 +40,9 @@ class RgbNet(nn.Sequential):
             return super().forward(x)
         from .mlp_hip import mlp_forward
         return mlp_forward(x, self.linears(), self.compute_dtype)
+
+    def forward_padded(self, x):
+        """forward() of rows already padded to the kernels' tile width (64 for the
+        42-wide fine input): x [M, 64] float32, zero past in_features."""
+        from .mlp_hip import mlp_forward
+        return mlp_forward(x, self.linears(), self.compute_dtype, padded=True)

@@ -39,11 +39,15 @@ class _MlpFn(torch.autograd.Function):
         M, din = x.shape
         dev = x.device
         st = stream_of(x)
-        kin = _pad_width(din)
+        kin = _pad_width(Ws[0].shape[1])
         kt0 = kin // 32
         adt = torch.bfloat16 if bf16 else torch.float32
-        X = torch.zeros((M, kin), dtype=adt, device=dev)
-        X[:, :din] = x.detach()
+        if din == kin and x.is_contiguous():
+            # already padded to the tile width (columns past in_features meet zero
+            # weights in the packed operand)
+            X = x.detach().to(adt)
+        else:
+            X = torch.nn.functional.pad(x.detach(), (0, kin - din)).to(adt).contiguous()
         hs = []
         h_in, kt = X, kt0
         nlin = len(Ws)
@@ -109,8 +113,12 @@ class _MlpFn(torch.autograd.Function):
         return (dX, None, *grads)
 
 
-def mlp_forward(x, linears, compute_dtype=torch.float32):
-    """logits = rgbnet(x) on the MFMA kernels (float32 or bfloat16 compute)."""
+def mlp_forward(x, linears, compute_dtype=torch.float32, padded=False):
+    """logits = rgbnet(x) on the MFMA kernels (float32 or bfloat16 compute).
+
+    padded=True: x [M, W] float32 rows already padded to the kernels' tile width W
+    (64 or 96, the smallest >= in_features) with finite values past in_features; they
+    are read in place and the gradient has the same [M, W] shape."""
     if not x.is_cuda:
         raise RuntimeError("the rgbnet kernels need CUDA (HIP) tensors")
     if len(linears) < 3 or any(l.out_features != WIDTH for l in linears[:-1]) or \
@@ -120,6 +128,14 @@ def mlp_forward(x, linears, compute_dtype=torch.float32):
     params = []
     for l in linears:
         params += [l.weight, l.bias]
-    x2 = x.reshape(-1, x.shape[-1]).float()
+    if padded:
+        kin = _pad_width(linears[0].in_features)
+        if x.dim() != 2 or x.shape[1] != kin or x.dtype != torch.float32 or not x.is_contiguous():
+            raise RuntimeError(f"padded rgbnet input must be contiguous float32 [M, {kin}]")
+        x2 = x
+    else:
+        if x.shape[-1] != linears[0].in_features:
+            raise RuntimeError(f"rgbnet input width {x.shape[-1]} != {linears[0].in_features}")
+        x2 = x.reshape(-1, x.shape[-1]).float()
     out = _MlpFn.apply(x2, bf16, *params)
     return out.reshape(*x.shape[:-1], 3)

@@ -110,10 +110,10 @@ class ResidentScene:
         rid = torch.empty(max(m, 1), dtype=torch.int32, device=self.dev)
         call("vx_fused_fine_gather", ptr(self.k0), self.k0_geom.data_ptr(), self.sdf_geom.data_ptr(),
              self.box.data_ptr(), self.params.data_ptr(), ptr(viewdirs), n, self.cap, self.cfg.view_pe,
-             ptr(ws.s4), ptr(ws.g4), ptr(ws.cnt), ptr(ws.off), XSTRIDE, ptr(X), ptr(P4), ptr(G4),
-             ptr(rid), st)
+             ptr(ws.s4), ptr(ws.g4), ptr(ws.stats), ptr(ws.cnt), ptr(ws.off), XSTRIDE, ptr(X),
+             ptr(P4), ptr(G4), ptr(rid), st)
         if m > 0:
-            logits = self.rgbnet(X[:m, :self.cfg.mlp_in]).contiguous()
+            logits = self.rgbnet.forward_padded(X[:m]).contiguous()
         else:
             logits = torch.zeros((1, 3), dtype=torch.float32, device=self.dev)
         call("vx_fused_fine_composite", n, ptr(ws.cnt), ptr(ws.off), ptr(logits), 3, ptr(P4),
