@@ -203,6 +203,17 @@ int vx_adam_upd(int dtype, void* param, const void* grad, void* exp_avg, void* e
 int vx_adam_step(int dtype, void* param, void* grad, void* exp_avg, void* exp_avg_sq,
                  const void* per_lr, int64_t n, int step, double lr, double beta1, double beta2,
                  double eps, int clear_grad, uint32_t* pages, void* stream);
+/* vx_adam_step (float32, clear_grad = 1, 16-byte aligned buffers) split in two passes
+ * over disjoint element sets, for overlap with other work.  pages holds
+ * 2 * ceil(n / 1024) + 1 words: the page flags, a count and a page list (count and list
+ * zero-initialised once).  Phase 1 updates the pages flagged zero (all-zero gradient, so
+ * it may run as soon as the flags are final, before the gradient is) from one block per
+ * CU, and lists the flagged pages; phase 2 updates the listed pages, clears their
+ * gradient and resets flags and count.  Phase 1 then phase 2 with the same step equal
+ * one vx_adam_step. */
+int vx_adam_step_split(int phase, float* param, float* grad, float* exp_avg, float* exp_avg_sq,
+                       const float* per_lr, int64_t n, int step, double lr, double beta1,
+                       double beta2, double eps, uint32_t* pages, void* stream);
 
 /* ---- per-sample ops ------------------------------------------------------------ */
 
@@ -249,12 +260,15 @@ int vx_fused_fine_fwd(const float* sdf, const int64_t* sdf_geom, const float* bo
  * off_incl[r]-cnt[r] .. off_incl[r]-1 and builds the rgbnet input rows
  * X [M, xstride] = [k0 | normal | view PE | 0] (lib/voxurf_fine.py:678-716; xstride a
  * multiple of 4, at most 64); P4 [M] = (x, y, z, weight), G4 [M] = (gradient, sdf),
- * rid [M] int32 ray ids. */
+ * rid [M] int32 ray ids.  k0_pages (may be NULL): set the page flag (vx_adam_step) of
+ * every k0_grad page the feature scatter of these rows can write, so that the k0 Adam
+ * pass over the other pages can start before the backward. */
 int vx_fused_fine_gather(const float* k0, const int64_t* k0_geom, const int64_t* sdf_geom,
                          const float* box, const float* params, const float* viewdirs,
                          int64_t n_rays, int cap, int n_freq, const void* s4, const void* g4,
                          const int* stats, const int64_t* cnt, const int64_t* off_incl,
-                         int xstride, float* X, void* P4, void* G4, int* rid, void* stream);
+                         int xstride, float* X, void* P4, void* G4, int* rid,
+                         uint32_t* k0_pages, void* stream);
 /* segment_coo compositing (lib/voxurf_fine.py:753-763), MSE loss per ray (run.py:604)
  * and the colour-side adjoints: dlogit [M,3], gw [M] (d loss / d weight), glast [N],
  * S [N] (sum of gw*w per ray), loss_r [N] (sum of squared errors).  With target NULL

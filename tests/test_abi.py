@@ -35,3 +35,30 @@ def test_binding_sets_argtypes_and_error_path():
     assert b"n_rays" in lib.vx_last_error()
     rc = lib.vx_raw2alpha(7, None, 0.0, 1.0, 5, None, None, None)
     assert rc == 4 and b"dtype" in lib.vx_last_error()
+
+
+def test_python_call_sites_match_header_arity():
+    """Every call("vx_...", ...) in the package, bench and entry point passes exactly the
+    number of arguments include/voxhip.h declares (a starred MaskCache triple counts 3)."""
+    import ast
+    import glob
+    decl = _lib.parse_header()
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    files = (glob.glob(os.path.join(root, "voxurf_amd", "*.py")) +
+             [os.path.join(root, "bench.py"), os.path.join(root, "__graft_entry__.py")])
+    bad = []
+    for f in files:
+        tree = ast.parse(open(f).read())
+        for node in ast.walk(tree):
+            if not (isinstance(node, ast.Call) and getattr(node.func, "id", None) == "call"
+                    and node.args and isinstance(node.args[0], ast.Constant)):
+                continue
+            name = node.args[0].value
+            n = 0
+            for a in node.args[1:]:
+                n += 3 if isinstance(a, ast.Starred) else 1
+            if name not in decl:
+                bad.append((os.path.basename(f), node.lineno, name, "not declared"))
+            elif n != len(decl[name][1]):
+                bad.append((os.path.basename(f), node.lineno, name, n, len(decl[name][1])))
+    assert not bad, bad

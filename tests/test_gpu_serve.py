@@ -55,7 +55,7 @@ def test_mix_equals_alone(dev, scenes):
     srv = Server(scenes, n_workers=3, batch_rays=65536, hbm_budget=1 << 30)
     for r in reqs:
         srv.submit(r)
-    srv.drain(len(reqs), timeout=300)
+    srv.drain(len(reqs), timeout=60)
     srv.close()
     for r in reqs:
         ref = alone(scenes, r, dev)
@@ -70,7 +70,7 @@ def test_budget_admission(dev, scenes):
     srv = Server(scenes, n_workers=2, batch_rays=32768, hbm_budget=1)  # admits one at a time
     for r in reqs:
         srv.submit(r)
-    srv.drain(len(reqs), timeout=300)
+    srv.drain(len(reqs), timeout=60)
     srv.close()
     for r in reqs:
         assert r.done_t >= r.admitted >= r.arrival

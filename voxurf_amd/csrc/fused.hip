@@ -214,6 +214,26 @@ __global__ void __launch_bounds__(256) k_fused_fwd(
   }
 }
 
+__device__ __forceinline__ void mark_page(uint32_t* pages, int64_t addr) {
+  pages[addr >> 10] = 1u;
+}
+
+// Flag the page(s) holding the C channels of trilinear corner `corner` (0..7, x outer,
+// z inner) of t: one run of C floats when channels are innermost, C strided words
+// otherwise.
+__device__ __forceinline__ void mark_corner_pages(uint32_t* pages, const GridDesc& g,
+                                                  const Tri<float>& t, int corner) {
+  const int x = t.x0 + (corner >> 2), y = t.y0 + ((corner >> 1) & 1), z = t.z0 + (corner & 1);
+  if (!in_grid<float>(x, y, z, g)) return;
+  const int64_t o = x * g.sx + y * g.sy + z * g.sz;
+  if (g.sc == 1) {
+    mark_page(pages, o);
+    mark_page(pages, o + g.C - 1);
+  } else {
+    for (int ch = 0; ch < g.C; ++ch) mark_page(pages, o + ch * g.sc);
+  }
+}
+
 // Compaction + rgbnet input.  One wavefront per ray takes its walk-range slots 64 at a
 // time, keeps the survivors (w > thres) at rows off[r] .. off[r] + cnt[r] and builds
 // X = [k0 (C) | normal (3) | view PE (3 + 6 nf) | 0 pad], four rows per pass with
@@ -224,11 +244,15 @@ __global__ void __launch_bounds__(256) k_fused_gather(
     FusedGeom G, const float* __restrict__ viewdirs, int64_t n_rays, int cap, int n_freq,
     const float4* __restrict__ s4, const float4* __restrict__ g4, const int* __restrict__ stats,
     const int64_t* __restrict__ cnt, const int64_t* __restrict__ off_incl, int xstride,
-    float* __restrict__ X, float4* __restrict__ P4, float4* __restrict__ G4, int* __restrict__ rid) {
+    float* __restrict__ X, float4* __restrict__ P4, float4* __restrict__ G4, int* __restrict__ rid,
+    uint32_t* __restrict__ pages) {
   const int lane = threadIdx.x & 63, grp = lane >> 4, cl = lane & 15;
   const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
   const int C = G.gk.C;
   const int npe = 3 + 6 * n_freq;
+  // channels innermost with 16-byte aligned corner rows: four channels per load
+  const bool vec4 = G.gk.sc == 1 && (C & 3) == 0 && ((G.gk.sx | G.gk.sy | G.gk.sz) & 3) == 0 &&
+                    (reinterpret_cast<uintptr_t>(G.k0) & 15) == 0;
   for (int64_t r = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6); r < n_rays;
        r += waves) {
     const int64_t c = cnt[r];
@@ -285,27 +309,38 @@ __global__ void __launch_bounds__(256) k_fused_gather(
         const int src = kth_set_bit(sm, on ? kk : 0);
         const float qx = __shfl(a.x, src, 64), qy = __shfl(a.y, src, 64), qz = __shfl(a.z, src, 64);
         const float gx = __shfl(b.x, src, 64), gy = __shfl(b.y, src, 64), gz = __shfl(b.z, src, 64);
-        if (on && 4 * cl < xstride) {
+        if (on) {
           Tri<float> t;
           t.init(world_to_index(qx, G.mn[0], G.mx[0] - G.mn[0], G.gk.X),
                  world_to_index(qy, G.mn[1], G.mx[1] - G.mn[1], G.gk.Y),
                  world_to_index(qz, G.mn[2], G.mx[2] - G.mn[2], G.gk.Z));
-          const float nrm = sqrtf(gx * gx + gy * gy + gz * gz) + 1e-5f;
-          float v[4];
+          if (pages && cl < 8) mark_corner_pages(pages, G.gk, t, cl);
+          if (4 * cl < xstride) {
+            const float nrm = sqrtf(gx * gx + gy * gy + gz * gz) + 1e-5f;
+            float v[4];
+            if (vec4 && 4 * cl + 3 < C) {
+              const float4 f = tri_gather4(G.k0, G.gk, t, 4 * cl);
+              v[0] = f.x;
+              v[1] = f.y;
+              v[2] = f.z;
+              v[3] = f.w;
+            } else {
 #pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const int col = 4 * cl + e;
-            float x = pev[e];
-            if (col < C) {
-              x = tri_gather(G.k0, G.gk, t, col);
-            } else if (col < C + 3) {
-              const float comp = (col == C) ? gx : (col == C + 1 ? gy : gz);
-              x = comp / nrm;
+              for (int e = 0; e < 4; ++e) {
+                const int col = 4 * cl + e;
+                float x = pev[e];
+                if (col < C) {
+                  x = tri_gather(G.k0, G.gk, t, col);
+                } else if (col < C + 3) {
+                  const float comp = (col == C) ? gx : (col == C + 1 ? gy : gz);
+                  x = comp / nrm;
+                }
+                v[e] = x;
+              }
             }
-            v[e] = x;
+            *reinterpret_cast<float4*>(X + (o + ks + kk) * xstride + 4 * cl) =
+                make_float4(v[0], v[1], v[2], v[3]);
           }
-          *reinterpret_cast<float4*>(X + (o + ks + kk) * xstride + 4 * cl) =
-              make_float4(v[0], v[1], v[2], v[3]);
         }
       }
       ks += ns;
@@ -370,9 +405,6 @@ __global__ void k_fused_composite(int64_t n_rays, const int64_t* __restrict__ cn
 
 // Feature-side adjoint per survivor: dX[:, 0:C] -> k0 grad (trilinear scatter);
 // dX[:, C:C+3] (normal) -> d grad through n = g / (|g| + 1e-5); result stored in dgf.
-__device__ __forceinline__ void mark_page(uint32_t* pages, int64_t addr) {
-  pages[addr >> 10] = 1u;
-}
 
 __global__ void __launch_bounds__(256) k_fused_feat_bwd(
     FusedGeom G, int64_t m, const float* __restrict__ dX, int dxs,
@@ -396,20 +428,7 @@ __global__ void __launch_bounds__(256) k_fused_feat_bwd(
              world_to_index(a.y, G.mn[1], G.mx[1] - G.mn[1], G.gk.Y),
              world_to_index(a.z, G.mn[2], G.mx[2] - G.mn[2], G.gk.Z));
       for (int ch = cl; ch < C; ch += 16) tri_scatter(k0_grad, G.gk, t, ch, dX[j * dxs + ch]);
-      if (pages && cl < 8) {
-        // flag every page corner cl's channels may occupy (one run of C floats when
-        // channels are innermost, C strided words otherwise)
-        const int x = t.x0 + (cl >> 2), y = t.y0 + ((cl >> 1) & 1), z = t.z0 + (cl & 1);
-        if (in_grid<float>(x, y, z, G.gk)) {
-          const int64_t o = x * G.gk.sx + y * G.gk.sy + z * G.gk.sz;
-          if (G.gk.sc == 1) {
-            mark_page(pages, o);
-            mark_page(pages, o + C - 1);
-          } else {
-            for (int ch = 0; ch < C; ++ch) mark_page(pages, o + ch * G.gk.sc);
-          }
-        }
-      }
+      if (pages && cl < 8) mark_corner_pages(pages, G.gk, t, cl);
     }
     // adjoint of the normal n = g / (|g| + 1e-5): one sample per lane
     const int64_t j = base + lane;
@@ -426,7 +445,9 @@ __global__ void __launch_bounds__(256) k_fused_feat_bwd(
   }
 }
 
-// Backward, one wavefront per ray over its stored walk-range samples:
+// Backward over each ray's stored walk-range samples, four rays per wavefront (sixteen
+// lanes per ray, 16 samples per pass; a ray holds about a dozen walk-range samples, so
+// a whole wave per ray would leave most lanes idle):
 //   dalpha_i = gw_i T_i - back_i / (1 - alpha_i + 1e-10)   on the walk range,
 //   back_i = glast * last + S_r - sum_{survivors j <= i} gw_j w_j
 // (render_utils_kernel.cu:654-677 with the suffix sum written as total - prefix), then
@@ -437,19 +458,31 @@ __global__ void __launch_bounds__(256) k_fused_bwd(
     const int64_t* __restrict__ cnt, const int64_t* __restrict__ off_incl,
     const float* __restrict__ last, const float* __restrict__ gw, const float* __restrict__ glast,
     const float* __restrict__ S, const float* __restrict__ dgf, float* __restrict__ sdf_grad) {
-  const int lane = threadIdx.x & 63;
-  const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
-  for (int64_t r = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6); r < n_rays;
-       r += waves) {
-    const float* vd = viewdirs + 3 * r;
-    const int c1 = stats[3 * r + 1];
-    const int64_t c_r = cnt[r];
-    const int64_t o = off_incl[r] - c_r;
-    const float base_back = glast[r] * last[r] + S[r];
+  constexpr int SEG = 16;
+  const int lane = threadIdx.x & 63, grp = lane / SEG, sl = lane % SEG;
+  const uint64_t seg_lt = sl ? ((1ull << sl) - 1ull) : 0ull;
+  const int64_t groups = (int64_t)gridDim.x * (blockDim.x >> 6) * (64 / SEG);
+  for (int64_t r0 = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * (64 / SEG);
+       r0 < n_rays; r0 += groups) {
+    const int64_t r = r0 + grp;
+    const bool ray_ok = r < n_rays;
+    const float* vd = viewdirs + 3 * (ray_ok ? r : 0);
+    const int c1 = ray_ok ? stats[3 * r + 1] : 0;
+    const int64_t c_r = ray_ok ? cnt[r] : 0;
+    const int64_t o = ray_ok ? off_incl[r] - c_r : 0;
+    const float base_back = ray_ok ? glast[r] * last[r] + S[r] : 0.f;
+    // every lane runs the same number of passes (the largest c1 of the four rays) so the
+    // segment shuffles below always see their whole segment active
+    int passes = (c1 + SEG - 1) / SEG;
+#pragma unroll
+    for (int off = SEG; off < 64; off <<= 1) {
+      const int t = __shfl_xor(passes, off, 64);
+      passes = t > passes ? t : passes;
+    }
     float prefix = 0.f;
     int k_surv = 0;
-    for (int base = 0; base < c1; base += 64) {
-      const int k = base + lane;
+    for (int ps = 0; ps < passes; ++ps) {
+      const int k = ps * SEG + sl;
       const bool valid = k < c1;
       SlotSample ss;
       bool surv = false;
@@ -459,9 +492,9 @@ __global__ void __launch_bounds__(256) k_fused_bwd(
       } else {
         ss.w = 0.f;
       }
-      const uint64_t sm = __ballot(surv);
+      const uint64_t sm = (__ballot(surv) >> (grp * SEG)) & 0xFFFFull;
       float gwl = 0.f, dg[3] = {0.f, 0.f, 0.f};
-      const int64_t kk = k_surv + __popcll(sm & lanemask_lt(lane));
+      const int64_t kk = k_surv + __popcll(sm & seg_lt);
       if (surv && kk < c_r) {
         const int64_t j = o + kk;
         gwl = gw[j];
@@ -470,16 +503,16 @@ __global__ void __launch_bounds__(256) k_fused_bwd(
         dg[2] = dgf[3 * j + 2];
       }
       k_surv += __popcll(sm);
-      // inclusive prefix of gw*w over survivors within the chunk
+      // inclusive prefix of gw*w over the survivors of this pass
       const float c = gwl * ss.w;
       float incl = c;
 #pragma unroll
-      for (int off = 1; off < 64; off <<= 1) {
-        const float u = __shfl_up(incl, off, 64);
-        if (lane >= off) incl += u;
+      for (int off = 1; off < SEG; off <<= 1) {
+        const float u = __shfl_up(incl, off, SEG);
+        if (sl >= off) incl += u;
       }
       const float back = base_back - (prefix + incl);
-      prefix += __shfl(incl, 63, 64);
+      prefix += __shfl(incl, SEG - 1, SEG);
       if (valid) {
         const double den = (double)(1.f - ss.alpha) + 1e-10;
         const float da = (float)((double)(gwl * ss.myT) - (double)back / den);
@@ -589,7 +622,8 @@ int vx_fused_fine_gather(const float* k0, const int64_t* k0_geom, const int64_t*
                          const float* box, const float* params, const float* viewdirs,
                          int64_t n_rays, int cap, int n_freq, const void* s4, const void* g4,
                          const int* stats, const int64_t* cnt, const int64_t* off_incl,
-                         int xstride, float* X, void* P4, void* G4, int* rid, void* stream) {
+                         int xstride, float* X, void* P4, void* G4, int* rid,
+                         uint32_t* k0_pages, void* stream) {
   FusedGeom G;
   if (int st = fused_geom(G, nullptr, sdf_geom, k0, k0_geom, box, params)) return st;
   VX_CHECK_ARG(G.gk.C + 3 + 3 + 6 * n_freq <= xstride && xstride <= 64 && xstride % 4 == 0,
@@ -598,7 +632,7 @@ int vx_fused_fine_gather(const float* k0, const int64_t* k0_geom, const int64_t*
   if (n_rays == 0) return VX_OK;
   k_fused_gather<<<grid_for(n_rays, 4, 1 << 16), 256, 0, (hipStream_t)stream>>>(
       G, viewdirs, n_rays, cap, n_freq, (const float4*)s4, (const float4*)g4, stats, cnt,
-      off_incl, xstride, X, (float4*)P4, (float4*)G4, rid);
+      off_incl, xstride, X, (float4*)P4, (float4*)G4, rid, k0_pages);
   return launch_status("fused_fine_gather");
 }
 
@@ -637,7 +671,7 @@ int vx_fused_fine_bwd(const float* sdf, const int64_t* sdf_geom, const float* bo
   if (int st = fused_geom(G, sdf, sdf_geom, nullptr, nullptr, box, params)) return st;
   VX_CHECK_ARG(cap >= 1, "fused_fine_bwd: cap < 1");
   if (n_rays == 0) return VX_OK;
-  k_fused_bwd<<<grid_for(n_rays, 4, 1 << 16), 256, 0, (hipStream_t)stream>>>(
+  k_fused_bwd<<<grid_for(n_rays, 16, 1 << 16), 256, 0, (hipStream_t)stream>>>(
       G, viewdirs, n_rays, cap, (const float4*)s4, (const float4*)g4, stats, cnt, off_incl, last,
       gw, glast, S, dgf, sdf_grad);
   return launch_status("fused_fine_bwd");

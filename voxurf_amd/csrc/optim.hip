@@ -5,6 +5,8 @@
 // zero-fill of the gradient buffers.
 #include <math.h>
 
+#include <algorithm>
+
 #include "common.h"
 
 namespace vx {
@@ -38,6 +40,7 @@ struct AdamCoef {
 //   denom = sqrt(v) / sqrt(1-b2^t) + eps;  p += -(lr/(1-b1^t)) * (m [* per_lr]) / denom
 __device__ __forceinline__ void adam_elem(float& p, float g, float& m, float& v, float plr,
                                           const AdamCoef& c) {
+#pragma clang fp contract(off)
   m = m * c.b1 + c.omb1 * g;
   v = v * c.b2 + (c.omb2 * g) * g;
   const float denom = sqrtf(v) / c.sqrt_bc2 + c.eps;
@@ -59,32 +62,57 @@ __device__ __forceinline__ void stnt(float4* p, float4 v) {
   __builtin_nontemporal_store(w, reinterpret_cast<nt_f4*>(p));
 }
 
+// Vector loop: each thread takes kAdamUnroll 16-byte vectors per array per iteration,
+// blockDim apart, with every load issued before the first store (more bytes in flight
+// per wave; non-temporal accesses keep the sweep from evicting the L2/MALL working set).
+constexpr int kAdamUnroll = 4;
+
 __global__ void __launch_bounds__(256) k_adam_step_f32(float* __restrict__ p, float* __restrict__ g,
                                                        float* __restrict__ m, float* __restrict__ v,
                                                        const float* __restrict__ perlr, int64_t n,
                                                        AdamCoef c, int clear, int vec,
                                                        const uint32_t* __restrict__ pages) {
+  constexpr int U = kAdamUnroll;
   const int64_t n4 = vec ? (n >> 2) : 0;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   float4* p4 = reinterpret_cast<float4*>(p);
   float4* g4 = reinterpret_cast<float4*>(g);
   float4* m4 = reinterpret_cast<float4*>(m);
   float4* v4 = reinterpret_cast<float4*>(v);
   const float4* l4 = reinterpret_cast<const float4*>(perlr);
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n4; i += stride) {
-    const bool live = !pages || pages[(i << 2) >> kPageShift] != 0;
-    float4 pp = ldnt(p4 + i), mm = ldnt(m4 + i), vv = ldnt(v4 + i);
-    float4 gg = live ? ldnt(g4 + i) : make_float4(0.f, 0.f, 0.f, 0.f);
-    const float4 ll = perlr ? l4[i] : make_float4(1.f, 1.f, 1.f, 1.f);
-    adam_elem(pp.x, gg.x, mm.x, vv.x, ll.x, c);
-    adam_elem(pp.y, gg.y, mm.y, vv.y, ll.y, c);
-    adam_elem(pp.z, gg.z, mm.z, vv.z, ll.z, c);
-    adam_elem(pp.w, gg.w, mm.w, vv.w, ll.w, c);
-    stnt(p4 + i, pp);
-    stnt(m4 + i, mm);
-    stnt(v4 + i, vv);
-    if (clear && live) stnt(g4 + i, make_float4(0.f, 0.f, 0.f, 0.f));
+  const int64_t vstride = (int64_t)gridDim.x * blockDim.x * U;
+  for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x * U + threadIdx.x; i0 < n4; i0 += vstride) {
+    float4 pp[U], mm[U], vv[U], gg[U], ll[U];
+    bool live[U], act[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t i = i0 + (int64_t)u * blockDim.x;
+      live[u] = false;
+      act[u] = i < n4;
+      if (act[u]) {
+        live[u] = !pages || pages[(i << 2) >> kPageShift] != 0;
+        pp[u] = ldnt(p4 + i);
+        mm[u] = ldnt(m4 + i);
+        vv[u] = ldnt(v4 + i);
+        gg[u] = live[u] ? ldnt(g4 + i) : make_float4(0.f, 0.f, 0.f, 0.f);
+        ll[u] = perlr ? l4[i] : make_float4(1.f, 1.f, 1.f, 1.f);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t i = i0 + (int64_t)u * blockDim.x;
+      if (act[u]) {
+        adam_elem(pp[u].x, gg[u].x, mm[u].x, vv[u].x, ll[u].x, c);
+        adam_elem(pp[u].y, gg[u].y, mm[u].y, vv[u].y, ll[u].y, c);
+        adam_elem(pp[u].z, gg[u].z, mm[u].z, vv[u].z, ll[u].z, c);
+        adam_elem(pp[u].w, gg[u].w, mm[u].w, vv[u].w, ll[u].w, c);
+        stnt(p4 + i, pp[u]);
+        stnt(m4 + i, mm[u]);
+        stnt(v4 + i, vv[u]);
+        if (clear && live[u]) stnt(g4 + i, make_float4(0.f, 0.f, 0.f, 0.f));
+      }
+    }
   }
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (n4 << 2) + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride) {
     const bool live = !pages || pages[i >> kPageShift] != 0;
     float pp = p[i], mm = m[i], vv = v[i];
@@ -93,6 +121,110 @@ __global__ void __launch_bounds__(256) k_adam_step_f32(float* __restrict__ p, fl
     m[i] = mm;
     v[i] = vv;
     if (clear && live) g[i] = 0.f;
+  }
+}
+
+// The two halves of vx_adam_step_split.  `pages` holds [flags (npages) | count | list
+// (npages)]: phase 1 updates the pages whose flag is zero and appends every flagged
+// page to the list; phase 2 updates the listed pages.  A page is 1024 elements = 256
+// float4, one per thread of a 256-thread block.
+template <int U>
+__global__ void __launch_bounds__(256) k_adam_dead_pages(float* __restrict__ p, float* __restrict__ m,
+                                                         float* __restrict__ v,
+                                                         const float* __restrict__ perlr, int64_t n,
+                                                         AdamCoef c, uint32_t* __restrict__ pages,
+                                                         int64_t npages) {
+  uint32_t* count = pages + npages;
+  uint32_t* list = pages + npages + 1;
+  const int64_t n4 = n >> 2;
+  float4* p4 = reinterpret_cast<float4*>(p);
+  float4* m4 = reinterpret_cast<float4*>(m);
+  float4* v4 = reinterpret_cast<float4*>(v);
+  const float4* l4 = reinterpret_cast<const float4*>(perlr);
+  const int tid = threadIdx.x;
+  for (int64_t pg0 = (int64_t)blockIdx.x * U; pg0 < npages; pg0 += (int64_t)gridDim.x * U) {
+    bool dead[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) dead[u] = pg0 + u < npages && pages[pg0 + u] == 0u;
+    if (tid == 0) {
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (pg0 + u < npages && !dead[u]) list[atomicAdd(count, 1u)] = (uint32_t)(pg0 + u);
+    }
+    float4 pp[U], mm[U], vv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t i = (pg0 + u) * 256 + tid;
+      if (dead[u] && i < n4) {
+        pp[u] = ldnt(p4 + i);
+        mm[u] = ldnt(m4 + i);
+        vv[u] = ldnt(v4 + i);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t i = (pg0 + u) * 256 + tid;
+      if (dead[u] && i < n4) {
+        const float4 ll = perlr ? l4[i] : make_float4(1.f, 1.f, 1.f, 1.f);
+        adam_elem(pp[u].x, 0.f, mm[u].x, vv[u].x, ll.x, c);
+        adam_elem(pp[u].y, 0.f, mm[u].y, vv[u].y, ll.y, c);
+        adam_elem(pp[u].z, 0.f, mm[u].z, vv[u].z, ll.z, c);
+        adam_elem(pp[u].w, 0.f, mm[u].w, vv[u].w, ll.w, c);
+        stnt(p4 + i, pp[u]);
+        stnt(m4 + i, mm[u]);
+        stnt(v4 + i, vv[u]);
+      } else if (dead[u] && i == n4) {
+        // the n % 4 trailing elements (last page)
+        for (int64_t e = n4 << 2; e < n; ++e) {
+          float a = p[e], b = m[e], d = v[e];
+          adam_elem(a, 0.f, b, d, perlr ? perlr[e] : 1.f, c);
+          p[e] = a;
+          m[e] = b;
+          v[e] = d;
+        }
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_adam_live_pages(float* __restrict__ p, float* __restrict__ g,
+                                                         float* __restrict__ m, float* __restrict__ v,
+                                                         const float* __restrict__ perlr, int64_t n,
+                                                         AdamCoef c, const uint32_t* __restrict__ pages,
+                                                         int64_t npages) {
+  const uint32_t cnt = pages[npages];
+  const uint32_t* list = pages + npages + 1;
+  const int64_t n4 = n >> 2;
+  float4* p4 = reinterpret_cast<float4*>(p);
+  float4* g4 = reinterpret_cast<float4*>(g);
+  float4* m4 = reinterpret_cast<float4*>(m);
+  float4* v4 = reinterpret_cast<float4*>(v);
+  const float4* l4 = reinterpret_cast<const float4*>(perlr);
+  const int tid = threadIdx.x;
+  for (int64_t k = blockIdx.x; k < (int64_t)cnt; k += gridDim.x) {
+    const int64_t pg = list[k];
+    const int64_t i = pg * 256 + tid;
+    if (i < n4) {
+      float4 pp = ldnt(p4 + i), gg = ldnt(g4 + i), mm = ldnt(m4 + i), vv = ldnt(v4 + i);
+      const float4 ll = perlr ? l4[i] : make_float4(1.f, 1.f, 1.f, 1.f);
+      adam_elem(pp.x, gg.x, mm.x, vv.x, ll.x, c);
+      adam_elem(pp.y, gg.y, mm.y, vv.y, ll.y, c);
+      adam_elem(pp.z, gg.z, mm.z, vv.z, ll.z, c);
+      adam_elem(pp.w, gg.w, mm.w, vv.w, ll.w, c);
+      stnt(p4 + i, pp);
+      stnt(m4 + i, mm);
+      stnt(v4 + i, vv);
+      stnt(g4 + i, make_float4(0.f, 0.f, 0.f, 0.f));
+    } else if (i == n4) {
+      for (int64_t e = n4 << 2; e < n; ++e) {
+        float a = p[e], b = m[e], d = v[e];
+        adam_elem(a, g[e], b, d, perlr ? perlr[e] : 1.f, c);
+        p[e] = a;
+        m[e] = b;
+        v[e] = d;
+        g[e] = 0.f;
+      }
+    }
   }
 }
 
@@ -175,8 +307,8 @@ int vx_adam_step(int dtype, void* param, void* grad, void* exp_avg, void* exp_av
     c.sqrt_bc2 = (float)sqrt(bc2);
     c.eps = (float)eps;
     c.neg_step = (float)(-step_size);
-    const int64_t work = vec ? n / 4 + (n & 3) : n;
-    k_adam_step_f32<<<grid_for(work, 256, 8192), 256, 0, st>>>(
+    const int64_t work = vec ? cdiv(n / 4, kAdamUnroll) + (n & 3) : n;
+    k_adam_step_f32<<<grid_for(work, 256, 16384), 256, 0, st>>>(
         (float*)param, (float*)grad, (float*)exp_avg, (float*)exp_avg_sq, (const float*)per_lr,
         n, c, clear_grad, vec, pages);
     if (pages) {
@@ -192,6 +324,53 @@ int vx_adam_step(int dtype, void* param, void* grad, void* exp_avg, void* exp_av
     return VX_ERR_DTYPE;
   }
   return launch_status("adam_step");
+}
+
+int vx_adam_step_split(int phase, float* param, float* grad, float* exp_avg, float* exp_avg_sq,
+                       const float* per_lr, int64_t n, int step, double lr, double beta1,
+                       double beta2, double eps, uint32_t* pages, void* stream) {
+  VX_CHECK_ARG(phase == 1 || phase == 2, "adam_step_split: phase must be 1 or 2");
+  VX_CHECK_ARG(pages != nullptr, "adam_step_split: page flags required");
+  VX_CHECK_ARG(n >= 0, "adam_step_split: n < 0");
+  VX_CHECK_ARG(step >= 1, "adam_step_split: step must be >= 1");
+  VX_CHECK_ARG((((uintptr_t)param | (uintptr_t)grad | (uintptr_t)exp_avg | (uintptr_t)exp_avg_sq |
+                 (uintptr_t)per_lr) % 16) == 0,
+               "adam_step_split: buffers must be 16-byte aligned");
+  if (n == 0) return VX_OK;
+  const double bc1 = 1.0 - pow(beta1, (double)step);
+  const double bc2 = 1.0 - pow(beta2, (double)step);
+  const double step_size = lr / bc1;
+  hipStream_t st = (hipStream_t)stream;
+  AdamCoef c;
+  c.b1 = (float)beta1;
+  c.omb1 = (float)(1.0 - beta1);
+  c.b2 = (float)beta2;
+  c.omb2 = (float)(1.0 - beta2);
+  c.sqrt_bc2 = (float)sqrt(bc2);
+  c.eps = (float)eps;
+  c.neg_step = (float)(-step_size);
+  const int64_t np = (n + (1 << kPageShift) - 1) >> kPageShift;
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+    cus = 256;
+  if (phase == 1) {
+    // one block per CU: enough loads in flight to stream near HBM rate while leaving the
+    // CUs' remaining wave slots and all of LDS to the kernels this pass overlaps
+    // (4 pages per iteration keeps it under 128 VGPRs, so an rgbnet layer block -- two
+    // waves of ~176 VGPRs per SIMD -- still fits beside it)
+    constexpr int U = 4;
+    const int64_t blocks = std::min<int64_t>(cus, cdiv(np, U));
+    k_adam_dead_pages<U><<<(unsigned)blocks, 256, 0, st>>>(param, exp_avg, exp_avg_sq, per_lr, n, c,
+                                                          pages, np);
+  } else {
+    const int64_t blocks = std::min<int64_t>(4 * (int64_t)cus, np);
+    k_adam_live_pages<<<(unsigned)blocks, 256, 0, st>>>(param, grad, exp_avg, exp_avg_sq, per_lr, n,
+                                                        c, pages, np);
+    // reset the flags and the list count for the next step
+    k_zero_u32<<<grid_for(np + 1, 256), 256, 0, st>>>(pages, np + 1);
+  }
+  return launch_status("adam_step_split");
 }
 
 }  // extern "C"

@@ -134,6 +134,29 @@ __device__ __forceinline__ T tri_gather(const T* __restrict__ grid, const GridDe
   return acc;
 }
 
+// Four consecutive channels c0 .. c0+3 of a channels-innermost grid (channel stride 1,
+// 16-byte aligned rows): one 16-byte load per corner, each channel summed exactly as
+// tri_gather sums it.
+__device__ __forceinline__ float4 tri_gather4(const float* __restrict__ grid, const GridDesc& g,
+                                             const Tri<float>& t, int c0) {
+#pragma clang fp contract(off)
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int bx = 0; bx < 2; ++bx)
+    for (int by = 0; by < 2; ++by)
+      for (int bz = 0; bz < 2; ++bz) {
+        const int x = t.x0 + bx, y = t.y0 + by, z = t.z0 + bz;
+        if (in_grid<float>(x, y, z, g)) {
+          const float4 v = *reinterpret_cast<const float4*>(grid + c0 + x * g.sx + y * g.sy + z * g.sz);
+          const float w = t.w(bx, by, bz);
+          acc.x += v.x * w;
+          acc.y += v.y * w;
+          acc.z += v.z * w;
+          acc.w += v.w * w;
+        }
+      }
+  return acc;
+}
+
 template <typename T>
 __device__ __forceinline__ void atomic_add(T* p, T v) {
   unsafeAtomicAdd(p, v);

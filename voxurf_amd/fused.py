@@ -13,8 +13,10 @@ One step (run.py:600-659 for the fine model, lib/voxurf_fine.py:620-802):
   6. rgbnet backward
   7. vx_fused_fine_feat_bwd   k0 scatter, normal adjoint
   8. vx_fused_fine_bwd        per-ray backward of the SDF side over the stored walk range
-  9. Adam (utils.Adam semantics) over sdf, k0 and rgbnet, clearing the grid gradients;
-     the k0 pass runs on a second stream, concurrently with step 8
+  9. Adam (utils.Adam semantics) over sdf, k0 and rgbnet, clearing the grid gradients.
+     The k0 pass is split by the page flags the gather sets: the pages no survivor can
+     reach (all-zero gradient) are updated on a second stream right after step 3,
+     concurrently with steps 4-6; the flagged pages after step 7, concurrently with 8.
 
 Grids: sdf [1,1,R,R,R] contiguous; k0 [1,C,R,R,R] channels-last.  Gradient buffers
 persist across steps (cleared by the Adam pass instead of reallocated).
@@ -57,10 +59,12 @@ class FusedFineTrainer:
         self.k0 = k0.to(self.dev, torch.float32).contiguous(memory_format=torch.channels_last_3d)
         self.sdf_grad = torch.zeros_like(self.sdf)
         self.k0_grad = torch.zeros_like(self.k0)
-        # one flag per 1024-element page of k0_grad: set by the feature scatter, consumed
-        # and reset by Adam, which skips the gradient traffic of untouched pages
-        self.k0_pages = torch.zeros((self.k0.numel() + 1023) // 1024, dtype=torch.int32,
-                                    device=self.dev)
+        # one flag per 1024-element page of k0_grad, set by the gather for every page the
+        # feature scatter can write; followed by the count and list of flagged pages that
+        # the split Adam builds (vx_adam_step_split)
+        n_pages = (self.k0.numel() + 1023) // 1024
+        self.k0_pages = torch.zeros(2 * n_pages + 1, dtype=torch.int32, device=self.dev)
+        self.n_pages = n_pages
         self.adam = {
             "sdf": [torch.zeros_like(self.sdf), torch.zeros_like(self.sdf), 0, cfg.lr_sdf],
             "k0": [torch.zeros_like(self.k0), torch.zeros_like(self.k0), 0, cfg.lr_k0],
@@ -80,8 +84,12 @@ class FusedFineTrainer:
         self.last_counts = {}
         self.count_all = True
         self.side = torch.cuda.Stream(self.dev)
-        # False: the k0 Adam pass runs in order on the main stream (for per-kernel timing)
+        # False: the k0 Adam pass runs in one piece, in order on the main stream
         self.overlap_adam = True
+        # a list: (start, end) events of the k0 Adam sweep (phase 1, or the whole pass when
+        # not overlapped) of every step are appended to it
+        self.adam_events = None
+        self.last_live_pages = None
 
     mask_cache = None
 
@@ -110,20 +118,28 @@ class FusedFineTrainer:
         self.loss = torch.empty(1, dtype=torch.float32, device=d)
         self._n = n
 
-    def _adam(self, name, p, g, profile=False, pages=None):
-        m, v, step, lr = self.adam[name]
-        step += 1
-        self.adam[name][2] = step
+    def _adam(self, name, p, g, pages=None, phase=0, step=None, timed=False):
+        """utils.Adam on one grid (betas (0.9, 0.99), eps 1e-8), clearing its gradient.
+        phase 1/2: the two halves of vx_adam_step_split (pages required); `step` is the
+        step count both halves share."""
+        m, v, _, lr = self.adam[name]
+        if step is None:
+            step = self.adam[name][2] + 1
+            self.adam[name][2] = step
         st = stream_of(p)
-        if profile:
+        timed = timed and self.adam_events is not None
+        if timed:
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record()
-        call("vx_adam_step", 0, ptr(p), ptr(g), ptr(m), ptr(v), None, p.numel(), step, float(lr),
-             0.9, 0.99, 1e-8, 1, ptr(pages), st)
-        if profile:
+        if phase == 0:
+            call("vx_adam_step", 0, ptr(p), ptr(g), ptr(m), ptr(v), None, p.numel(), step,
+                 float(lr), 0.9, 0.99, 1e-8, 1, ptr(pages), st)
+        else:
+            call("vx_adam_step_split", phase, ptr(p), ptr(g), ptr(m), ptr(v), None, p.numel(),
+                 step, float(lr), 0.9, 0.99, 1e-8, ptr(pages), st)
+        if timed:
             e1.record()
-            return (e0, e1)
-        return None
+            self.adam_events.append((e0, e1))
 
     # --- one step -----------------------------------------------------------------
     def forward(self, rays_o, rays_d, viewdirs):
@@ -146,7 +162,7 @@ class FusedFineTrainer:
              self.sdf_geom.data_ptr(), self.box.data_ptr(), self.params.data_ptr(),
              ptr(viewdirs), n, self.cap, self.cfg.view_pe, ptr(self.s4), ptr(self.g4),
              ptr(self.stats), ptr(self.cnt), ptr(self.off), XSTRIDE, ptr(self.X), ptr(self.P4),
-             ptr(self.G4), ptr(self.rid), st)
+             ptr(self.G4), ptr(self.rid), ptr(self.k0_pages), st)
         return m
 
     def step(self, rays_o, rays_d, viewdirs, target, profile=False):
@@ -154,6 +170,20 @@ class FusedFineTrainer:
         n = rays_o.shape[0]
         m = self.forward(rays_o, rays_d, viewdirs)
         st = stream_of(self.sdf)
+        main = torch.cuda.current_stream(self.dev)
+        k0_step = self.adam["k0"][2] + 1
+        self.adam["k0"][2] = k0_step
+        if self.overlap_adam:
+            # The gather has flagged every k0 page the feature scatter can write; the
+            # others have an all-zero gradient this step, so their Adam update (most of
+            # the 256^3 x 12 sweep, HBM-bound) runs now on a second stream, beside the
+            # MFMA-bound rgbnet passes.
+            flags_ready = torch.cuda.Event()
+            flags_ready.record(main)
+            with torch.cuda.stream(self.side):
+                self.side.wait_event(flags_ready)
+                self._adam("k0", self.k0, self.k0_grad, pages=self.k0_pages, phase=1, step=k0_step,
+                           timed=True)
         din = cfg.mlp_in
         xin = self.X[:m].detach().requires_grad_(True)
         logits = self.rgbnet.forward_padded(xin)
@@ -174,22 +204,23 @@ class FusedFineTrainer:
         dgf = torch.empty((max(m, 1), 3), dtype=torch.float32, device=self.dev)
         call("vx_fused_fine_feat_bwd", self.k0_geom.data_ptr(), self.sdf_geom.data_ptr(),
              self.box.data_ptr(), self.params.data_ptr(), m, ptr(dX), dX.stride(0), ptr(self.P4),
-             ptr(self.G4), ptr(self.k0_grad), ptr(dgf), ptr(self.k0_pages), st)
-        # The k0 Adam pass (HBM-bound, the largest kernel of the step) depends only on
-        # the feature scatter above: it runs on a second stream, overlapping the SDF-side
-        # backward (L2/atomic-bound) and the remaining optimiser passes.
-        main = torch.cuda.current_stream(self.dev)
+             ptr(self.G4), ptr(self.k0_grad), ptr(dgf), None, st)
         if profile:
             # snapshot before any optimiser pass clears a gradient
             k0_grad_snapshot = self.k0_grad.clone()
-        k0_stream = self.side if self.overlap_adam else main
-        grads_ready = torch.cuda.Event()
-        grads_ready.record(main)
-        with torch.cuda.stream(k0_stream):
-            k0_stream.wait_event(grads_ready)
-            ev = self._adam("k0", self.k0, self.k0_grad, profile=profile, pages=self.k0_pages)
-            k0_done = torch.cuda.Event()
-            k0_done.record(k0_stream)
+            self.last_live_pages = int(self.k0_pages[:self.n_pages].count_nonzero())
+        if self.overlap_adam:
+            # the flagged pages, once the feature scatter has finished
+            grads_ready = torch.cuda.Event()
+            grads_ready.record(main)
+            with torch.cuda.stream(self.side):
+                self.side.wait_event(grads_ready)
+                self._adam("k0", self.k0, self.k0_grad, pages=self.k0_pages, phase=2, step=k0_step)
+                k0_done = torch.cuda.Event()
+                k0_done.record(self.side)
+        else:
+            self._adam("k0", self.k0, self.k0_grad, pages=self.k0_pages, phase=0, step=k0_step,
+                       timed=True)
         call("vx_fused_fine_bwd", ptr(self.sdf), self.sdf_geom.data_ptr(), self.box.data_ptr(),
              self.params.data_ptr(), ptr(viewdirs), n, self.cap, ptr(self.s4), ptr(self.g4),
              ptr(self.stats), ptr(self.cnt), ptr(self.off), ptr(self.last), ptr(gw),
@@ -200,11 +231,11 @@ class FusedFineTrainer:
         self._adam("sdf", self.sdf, self.sdf_grad)
         self.mlp_opt.step()
         call("vx_sum_f32", ptr(self.loss_r), n, ptr(self.loss), st)
-        # the next step's gather reads k0
-        main.wait_event(k0_done)
+        if self.overlap_adam:
+            # the next step's gather reads k0
+            main.wait_event(k0_done)
         if profile:
             torch.cuda.synchronize()
-            self.last_profile = {"adam_k0_ms": ev[0].elapsed_time(ev[1])}
             s = self.stats[:n].sum(0).tolist()
             self.last_counts = {"m0": int(s[2]), "m0_processed": int(s[0]), "m1b": int(s[1]),
                                 "m2": m}

@@ -111,7 +111,7 @@ class ResidentScene:
         call("vx_fused_fine_gather", ptr(self.k0), self.k0_geom.data_ptr(), self.sdf_geom.data_ptr(),
              self.box.data_ptr(), self.params.data_ptr(), ptr(viewdirs), n, self.cap, self.cfg.view_pe,
              ptr(ws.s4), ptr(ws.g4), ptr(ws.stats), ptr(ws.cnt), ptr(ws.off), XSTRIDE, ptr(X),
-             ptr(P4), ptr(G4), ptr(rid), st)
+             ptr(P4), ptr(G4), ptr(rid), None, st)
         if m > 0:
             logits = self.rgbnet.forward_padded(X[:m]).contiguous()
         else:
@@ -180,6 +180,7 @@ class Server:
         self.cv = threading.Condition()
         self.done = []
         self.stop = False
+        self.error = None            # first exception raised in a worker
         cap = max(s.cap for s in scenes)
         self.workspaces = [Workspace(self.dev, cap) for _ in range(n_workers)]
         for ws in self.workspaces:
@@ -244,6 +245,18 @@ class Server:
         self.done.append(r)
 
     def _worker(self, ws):
+        # an exception in a worker stops the server and is re-raised by drain(), so a
+        # failing request never leaves its waiters blocked
+        try:
+            self._work(ws)
+        except BaseException as e:
+            with self.cv:
+                if self.error is None:
+                    self.error = e
+                self.stop = True
+                self.cv.notify_all()
+
+    def _work(self, ws):
         while True:
             with self.cv:
                 item = None
@@ -287,6 +300,8 @@ class Server:
         t0 = time.perf_counter()
         with self.cv:
             while len(self.done) < n_expected:
+                if self.error is not None:
+                    raise RuntimeError("a serving worker failed") from self.error
                 if time.perf_counter() - t0 > timeout:
                     raise TimeoutError(f"{len(self.done)}/{n_expected} requests done")
                 self.cv.wait(0.01)

@@ -311,7 +311,7 @@ def serve(args):
         while time.perf_counter() - t0 < a:
             time.sleep(0.0002)
         srv.submit(r)
-    srv.drain(len(reqs), timeout=1200)
+    srv.drain(len(reqs), timeout=150)
     srv.close()
     batched = stats(reqs, t0, max(r.done_t for r in reqs))
     out_b = {r.rid: r.out for r in reqs}
