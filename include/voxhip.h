@@ -305,6 +305,14 @@ int vx_mlp_pack(int bf16, const float* W, int n_out, int k_in, int kt, int nto, 
                 void* out, void* stream);
 /* Element count of a packed operand. */
 int64_t vx_mlp_pack_elems(int kt, int nto);
+/* All packed operands of an rgbnet in one launch: w_ptrs is a host array of the
+ * n_hidden hidden layers' device weight pointers (layer 0 [192, k_in0], the others
+ * [192, 192]).  For each layer li in order, the forward operand (kt = 2 or 3 for layer
+ * 0 -- k_in0 <= 64 or <= 96 -- else 6; nto = 6) then the transposed backward operand
+ * (kt = 6, nto = the forward kt), concatenated. */
+int64_t vx_mlp_pack_net_elems(int n_hidden, int k_in0);
+int vx_mlp_pack_net(int bf16, int n_hidden, const int64_t* w_ptrs, int k_in0, void* out,
+                    void* stream);
 /* One hidden layer: out = relu(W in + b); with last = 1 also logits [M,3] = W3 out + b3. */
 int vx_mlp_layer_fwd(int bf16, int kt, int last, const void* in, int in_stride, int64_t M,
                      const void* wpk, const float* bias, void* out, int out_stride,
@@ -329,8 +337,8 @@ int vx_mlp_last_bwd(int bf16, const float* dlogit, const void* H, int hs, int64_
 /* y = bfloat16(x), n elements. */
 int vx_f32_to_bf16(const float* x, int64_t n, void* y, void* stream);
 
-/* *out = sum of x[0..n) (float32, device scalar). */
-int vx_sum_f32(const float* x, int64_t n, float* out, void* stream);
+/* *out = scale * (sum of x[0..n)) (float32, device scalar; fixed summation order). */
+int vx_sum_f32(const float* x, int64_t n, float scale, float* out, void* stream);
 
 #ifdef __cplusplus
 }

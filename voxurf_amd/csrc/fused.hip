@@ -527,13 +527,20 @@ __global__ void __launch_bounds__(256) k_fused_bwd(
   }
 }
 
-__global__ void k_sum_f32(const float* __restrict__ x, int64_t n, float* __restrict__ out) {
+// out[0] = scale * sum x[0..n): one block, fixed summation order (deterministic).
+__global__ void __launch_bounds__(1024) k_sum_f32(const float* __restrict__ x, int64_t n, float scale,
+                                                  float* __restrict__ out) {
+  __shared__ float part[16];
   float s = 0.f;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    s += x[i];
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) s += x[i];
   s = wave_sum_f32(s);
-  if ((threadIdx.x & 63) == 0) unsafeAtomicAdd(out, s);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float t = 0.f;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += part[w];
+    out[0] = t * scale;
+  }
 }
 
 }  // namespace vx
@@ -677,12 +684,9 @@ int vx_fused_fine_bwd(const float* sdf, const int64_t* sdf_geom, const float* bo
   return launch_status("fused_fine_bwd");
 }
 
-int vx_sum_f32(const float* x, int64_t n, float* out, void* stream) {
+int vx_sum_f32(const float* x, int64_t n, float scale, float* out, void* stream) {
   VX_CHECK_ARG(n >= 0, "sum: n < 0");
-  hipStream_t st = (hipStream_t)stream;
-  vx_zero(VX_F32, out, 1, stream);
-  if (n == 0) return launch_status("sum");
-  k_sum_f32<<<grid_for(n, 256, 1024), 256, 0, st>>>(x, n, out);
+  k_sum_f32<<<1, 1024, 0, (hipStream_t)stream>>>(x, n, scale, out);
   return launch_status("sum");
 }
 

@@ -33,43 +33,87 @@ __host__ __device__ constexpr int dfeat(int t, int v, int h) {
 //   fwd: A[i = 32 to + lane%32][k = dfeat(s/16, s%16, lane/32)] = Wt[i][k]
 //   bwd: A[i = 32 to + lane%32][k = ...] = Wt[k][i]        (W transposed)
 // with zero outside [0, n_out) x [0, k_in).  S = 16 * KT reduction steps.
+__device__ __forceinline__ float pack_f32_value(const float* __restrict__ Wt, int n_out, int k_in,
+                                                int KT, int transpose, int64_t idx, int64_t& dst) {
+  const int S = 16 * KT;
+  const int lane = (int)(idx & 63), s = (int)((idx >> 6) % S), to = (int)((idx >> 6) / S);
+  const int i = 32 * to + (lane & 31);
+  const int k = dfeat(s >> 4, s & 15, lane >> 5);
+  float v = 0.f;
+  if (!transpose) {
+    if (i < n_out && k < k_in) v = Wt[i * k_in + k];
+  } else {
+    // A[i][k] = Wt[k][i]: i indexes the layer's inputs (< k_in), k its outputs (< n_out)
+    if (k < n_out && i < k_in) v = Wt[k * k_in + i];
+  }
+  dst = (((int64_t)to * (S / 4) + (s >> 2)) * 64 + lane) * 4 + (s & 3);
+  return v;
+}
+
 __global__ void k_pack_f32(const float* __restrict__ Wt, int n_out, int k_in, int KT, int NTO,
                            int transpose, float* __restrict__ out) {
-  const int S = 16 * KT;
-  const int total = NTO * S * 64;
-  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += gridDim.x * blockDim.x) {
-    const int lane = idx & 63, s = (idx >> 6) % S, to = (idx >> 6) / S;
-    const int i = 32 * to + (lane & 31);
-    const int k = dfeat(s >> 4, s & 15, lane >> 5);
-    float v = 0.f;
-    if (!transpose) {
-      if (i < n_out && k < k_in) v = Wt[i * k_in + k];
-    } else {
-      // A[i][k] = Wt[k][i]: i indexes the layer's inputs (< k_in), k its outputs (< n_out)
-      if (k < n_out && i < k_in) v = Wt[k * k_in + i];
-    }
-    out[(((to * (S / 4)) + (s >> 2)) * 64 + lane) * 4 + (s & 3)] = v;
+  const int64_t total = (int64_t)NTO * 16 * KT * 64;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    int64_t dst;
+    const float v = pack_f32_value(Wt, n_out, k_in, KT, transpose, idx, dst);
+    out[dst] = v;
   }
 }
 
 // Packed A operand, BF16 (K = 16 per MFMA, 8 values per lane): element
 // ((to * S + s) * 64 + lane) * 8 + e holds A[i = 32 to + lane%32][k] with
 // k = dfeat(s / 2, 8 (s % 2) + e, lane / 32), S = 2 * KT steps.
+__device__ __forceinline__ float pack_bf16_value(const float* __restrict__ Wt, int n_out, int k_in,
+                                                 int KT, int transpose, int64_t idx) {
+  const int S = 2 * KT;
+  const int e = (int)(idx & 7), lane = (int)((idx >> 3) & 63), s = (int)((idx >> 9) % S),
+            to = (int)((idx >> 9) / S);
+  const int i = 32 * to + (lane & 31);
+  const int k = dfeat(s >> 1, 8 * (s & 1) + e, lane >> 5);
+  (void)e;
+  float v = 0.f;
+  if (!transpose) {
+    if (i < n_out && k < k_in) v = Wt[i * k_in + k];
+  } else {
+    if (k < n_out && i < k_in) v = Wt[k * k_in + i];
+  }
+  return v;
+}
+
 __global__ void k_pack_bf16(const float* __restrict__ Wt, int n_out, int k_in, int KT, int NTO,
                             int transpose, __bf16* __restrict__ out) {
-  const int S = 2 * KT;
-  const int total = NTO * S * 64 * 8;
-  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += gridDim.x * blockDim.x) {
-    const int e = idx & 7, lane = (idx >> 3) & 63, s = (idx >> 9) % S, to = (idx >> 9) / S;
-    const int i = 32 * to + (lane & 31);
-    const int k = dfeat(s >> 1, 8 * (s & 1) + e, lane >> 5);
-    float v = 0.f;
-    if (!transpose) {
-      if (i < n_out && k < k_in) v = Wt[i * k_in + k];
+  const int64_t total = (int64_t)NTO * 2 * KT * 64 * 8;
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x)
+    out[idx] = (__bf16)pack_bf16_value(Wt, n_out, k_in, KT, transpose, idx);
+}
+
+// Every packed operand of a network in one launch: segment blockIdx.y.
+struct PackSeg {
+  const float* W;
+  int64_t offset;  // elements into the output
+  int n_out, k_in, kt, nto, transpose;
+};
+constexpr int kMaxPackSegs = 8;
+struct PackNet {
+  PackSeg seg[kMaxPackSegs];
+};
+
+template <bool BF>
+__global__ void k_pack_net(PackNet net, void* __restrict__ out) {
+  const PackSeg sg = net.seg[blockIdx.y];
+  const int64_t total = (int64_t)sg.nto * 16 * sg.kt * 64;  // same count in both layouts
+  for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    if constexpr (BF) {
+      reinterpret_cast<__bf16*>(out)[sg.offset + idx] =
+          (__bf16)pack_bf16_value(sg.W, sg.n_out, sg.k_in, sg.kt, sg.transpose, idx);
     } else {
-      if (k < n_out && i < k_in) v = Wt[k * k_in + i];
+      int64_t dst;
+      const float v = pack_f32_value(sg.W, sg.n_out, sg.k_in, sg.kt, sg.transpose, idx, dst);
+      reinterpret_cast<float*>(out)[sg.offset + dst] = v;
     }
-    out[idx] = (__bf16)v;
   }
 }
 
@@ -469,44 +513,52 @@ __global__ void __launch_bounds__(256) k_mlp_wgrad(const float* __restrict__ dY,
 //   pass 2: out[r * k_real + c] = sum over g of red[g][r * k_pad + c], c < k_real.
 constexpr int kReduceGroups = 16;
 
-__global__ void k_reduce_pass1(const float* __restrict__ part, int nblk, int n,
-                               float* __restrict__ red) {
+// Two independent reductions (a weight slice and its bias slice) in the same two
+// launches: blockIdx.z selects the job.
+struct RedJob {
+  const float* part;
+  float* red;
+  float* out;
+  int n, rows, k_pad, k_real;
+};
+
+__global__ void k_reduce2_pass1(RedJob ja, RedJob jb, int nblk) {
+  const RedJob& j = blockIdx.z ? jb : ja;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int g = blockIdx.y;
-  if (i >= n) return;
+  if (i >= j.n) return;
   const int b0 = (int)((int64_t)nblk * g / kReduceGroups);
   const int b1 = (int)((int64_t)nblk * (g + 1) / kReduceGroups);
   float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
   int b = b0;
   for (; b + 4 <= b1; b += 4) {
-    s0 += part[(int64_t)b * n + i];
-    s1 += part[(int64_t)(b + 1) * n + i];
-    s2 += part[(int64_t)(b + 2) * n + i];
-    s3 += part[(int64_t)(b + 3) * n + i];
+    s0 += j.part[(int64_t)b * j.n + i];
+    s1 += j.part[(int64_t)(b + 1) * j.n + i];
+    s2 += j.part[(int64_t)(b + 2) * j.n + i];
+    s3 += j.part[(int64_t)(b + 3) * j.n + i];
   }
-  for (; b < b1; ++b) s0 += part[(int64_t)b * n + i];
-  red[(int64_t)g * n + i] = (s0 + s1) + (s2 + s3);
+  for (; b < b1; ++b) s0 += j.part[(int64_t)b * j.n + i];
+  j.red[(int64_t)g * j.n + i] = (s0 + s1) + (s2 + s3);
 }
 
-__global__ void k_reduce_pass2(const float* __restrict__ red, int n, int rows, int k_pad,
-                               int k_real, float* __restrict__ out) {
+__global__ void k_reduce2_pass2(RedJob ja, RedJob jb) {
+  const RedJob& j = blockIdx.z ? jb : ja;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= rows * k_real) return;
-  const int r = i / k_real, c = i % k_real;
+  if (i >= j.rows * j.k_real) return;
+  const int r = i / j.k_real, c = i % j.k_real;
   float s = 0.f;
 #pragma unroll
-  for (int g = 0; g < kReduceGroups; ++g) s += red[(int64_t)g * n + r * k_pad + c];
-  out[i] = s;
+  for (int g = 0; g < kReduceGroups; ++g) s += j.red[(int64_t)g * j.n + r * j.k_pad + c];
+  j.out[i] = s;
 }
 
-// Sum nblk slices [rows, k_pad] of `part` into out [rows, k_real]; `red` holds
+// out_a [rows_a, k_real_a] and out_b from nblk slices each; each job's `red` holds
 // kReduceGroups slices.
-static void reduce_slices(const float* part, int nblk, int rows, int k_pad, int k_real,
-                          float* red, float* out, hipStream_t st) {
-  const int n = rows * k_pad;
-  dim3 g1((unsigned)cdiv(n, 256), kReduceGroups);
-  k_reduce_pass1<<<g1, 256, 0, st>>>(part, nblk, n, red);
-  k_reduce_pass2<<<(unsigned)cdiv(rows * k_real, 256), 256, 0, st>>>(red, n, rows, k_pad, k_real, out);
+static void reduce_two(const RedJob& a, const RedJob& b, int nblk, hipStream_t st) {
+  const int n = a.n > b.n ? a.n : b.n;
+  const int o = a.rows * a.k_real > b.rows * b.k_real ? a.rows * a.k_real : b.rows * b.k_real;
+  k_reduce2_pass1<<<dim3((unsigned)cdiv(n, 256), kReduceGroups, 2), 256, 0, st>>>(a, b, nblk);
+  k_reduce2_pass2<<<dim3((unsigned)cdiv(o, 256), 1, 2), 256, 0, st>>>(a, b);
 }
 
 __global__ void k_f32_to_bf16(const float* __restrict__ x, int64_t n, __bf16* __restrict__ y) {
@@ -577,6 +629,57 @@ int vx_mlp_pack(int bf16, const float* W, int n_out, int k_in, int kt, int nto, 
 }
 
 int64_t vx_mlp_pack_elems(int kt, int nto) { return (int64_t)nto * kt * 16 * 64; }
+
+static int pack_net_segments(int n_hidden, int k_in0, const int64_t* w_ptrs, PackNet* net,
+                             int64_t* total) {
+  const int kt0 = k_in0 <= 64 ? 2 : 3;
+  int64_t off = 0;
+  for (int li = 0; li < n_hidden; ++li) {
+    const int kt = li == 0 ? kt0 : 6;
+    const int k_in = li == 0 ? k_in0 : kWidth;
+    for (int dir = 0; dir < 2; ++dir) {
+      PackSeg sg;
+      sg.W = w_ptrs ? reinterpret_cast<const float*>(w_ptrs[li]) : nullptr;
+      sg.offset = off;
+      sg.n_out = kWidth;
+      sg.k_in = k_in;
+      sg.kt = dir == 0 ? kt : 6;
+      sg.nto = dir == 0 ? 6 : kt;
+      sg.transpose = dir;
+      if (net) net->seg[2 * li + dir] = sg;
+      off += vx_mlp_pack_elems(sg.kt, sg.nto);
+    }
+  }
+  *total = off;
+  return VX_OK;
+}
+
+int64_t vx_mlp_pack_net_elems(int n_hidden, int k_in0) {
+  if (n_hidden < 1 || 2 * n_hidden > kMaxPackSegs || k_in0 < 1 || k_in0 > 96) return -1;
+  int64_t total = 0;
+  pack_net_segments(n_hidden, k_in0, nullptr, nullptr, &total);
+  return total;
+}
+
+int vx_mlp_pack_net(int bf16, int n_hidden, const int64_t* w_ptrs, int k_in0, void* out,
+                    void* stream) {
+  VX_CHECK_ARG(n_hidden >= 1 && 2 * n_hidden <= kMaxPackSegs, "mlp_pack_net: 1..%d hidden layers",
+               kMaxPackSegs / 2);
+  VX_CHECK_ARG(k_in0 >= 1 && k_in0 <= 96, "mlp_pack_net: input width must be 1..96");
+  PackNet net;
+  int64_t total = 0;
+  pack_net_segments(n_hidden, k_in0, w_ptrs, &net, &total);
+  int64_t biggest = 0;
+  for (int i = 0; i < 2 * n_hidden; ++i) {
+    const int64_t e = vx_mlp_pack_elems(net.seg[i].kt, net.seg[i].nto);
+    if (e > biggest) biggest = e;
+  }
+  dim3 grid(grid_for(biggest, 256), 2 * n_hidden);
+  hipStream_t st = (hipStream_t)stream;
+  if (bf16) k_pack_net<true><<<grid, 256, 0, st>>>(net, out);
+  else k_pack_net<false><<<grid, 256, 0, st>>>(net, out);
+  return launch_status("mlp_pack_net");
+}
 
 int vx_mlp_layer_fwd(int bf16, int kt, int last, const void* in, int in_stride, int64_t M,
                      const void* wpk, const float* bias, void* out, int out_stride,
@@ -657,8 +760,9 @@ int vx_mlp_wgrad(int bf16, int kt, int k_real, const float* dY, int dys, const v
   else if (kt == 3) { if (bf16) L(3, true); else L(3, false); }
   else { if (bf16) L(6, true); else L(6, false); }
 #undef L
-  reduce_slices(part, nblk, kWidth, K, k_real, part + (int64_t)nblk * kWidth * K, dW, st);
-  reduce_slices(partb, nblk, 1, kWidth, kWidth, partb + (int64_t)nblk * kWidth, db, st);
+  const RedJob jw{part, part + (int64_t)nblk * kWidth * K, dW, kWidth * K, kWidth, K, k_real};
+  const RedJob jb{partb, partb + (int64_t)nblk * kWidth, db, kWidth, 1, kWidth, kWidth};
+  reduce_two(jw, jb, nblk, st);
   return launch_status("mlp_wgrad");
 }
 
@@ -671,8 +775,9 @@ int vx_mlp_last_bwd(int bf16, const float* dlogit, const void* H, int hs, int64_
   const int64_t rpb = cdiv(M, nblk);
   if (bf16) k_mlp_last_bwd<true><<<nblk, kWidth, 0, st>>>(dlogit, H, hs, M, rpb, w3, dH, dhs, part, partb);
   else k_mlp_last_bwd<false><<<nblk, kWidth, 0, st>>>(dlogit, H, hs, M, rpb, w3, dH, dhs, part, partb);
-  reduce_slices(part, nblk, 3, kWidth, kWidth, part + (int64_t)nblk * 3 * kWidth, dW3, st);
-  reduce_slices(partb, nblk, 1, 3, 3, partb + (int64_t)nblk * 3, db3, st);
+  const RedJob jw{part, part + (int64_t)nblk * 3 * kWidth, dW3, 3 * kWidth, 3, kWidth, kWidth};
+  const RedJob jb{partb, partb + (int64_t)nblk * 3, db3, 3, 1, 3, 3};
+  reduce_two(jw, jb, nblk, st);
   return launch_status("mlp_last_bwd");
 }
 

@@ -201,28 +201,44 @@ __global__ void __launch_bounds__(256) k_adam_live_pages(float* __restrict__ p, 
   float4* v4 = reinterpret_cast<float4*>(v);
   const float4* l4 = reinterpret_cast<const float4*>(perlr);
   const int tid = threadIdx.x;
-  for (int64_t k = blockIdx.x; k < (int64_t)cnt; k += gridDim.x) {
-    const int64_t pg = list[k];
-    const int64_t i = pg * 256 + tid;
-    if (i < n4) {
-      float4 pp = ldnt(p4 + i), gg = ldnt(g4 + i), mm = ldnt(m4 + i), vv = ldnt(v4 + i);
-      const float4 ll = perlr ? l4[i] : make_float4(1.f, 1.f, 1.f, 1.f);
-      adam_elem(pp.x, gg.x, mm.x, vv.x, ll.x, c);
-      adam_elem(pp.y, gg.y, mm.y, vv.y, ll.y, c);
-      adam_elem(pp.z, gg.z, mm.z, vv.z, ll.z, c);
-      adam_elem(pp.w, gg.w, mm.w, vv.w, ll.w, c);
-      stnt(p4 + i, pp);
-      stnt(m4 + i, mm);
-      stnt(v4 + i, vv);
-      stnt(g4 + i, make_float4(0.f, 0.f, 0.f, 0.f));
-    } else if (i == n4) {
-      for (int64_t e = n4 << 2; e < n; ++e) {
-        float a = p[e], b = m[e], d = v[e];
-        adam_elem(a, g[e], b, d, perlr ? perlr[e] : 1.f, c);
-        p[e] = a;
-        m[e] = b;
-        v[e] = d;
-        g[e] = 0.f;
+  // four listed pages per block iteration: every load is issued before the first store
+  constexpr int U = 4;
+  for (int64_t k0 = (int64_t)blockIdx.x * U; k0 < (int64_t)cnt; k0 += (int64_t)gridDim.x * U) {
+    float4 pp[U], gg[U], mm[U], vv[U];
+    int64_t idx[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      idx[u] = -1;
+      if (k0 + u < (int64_t)cnt) idx[u] = (int64_t)list[k0 + u] * 256 + tid;
+      if (idx[u] >= 0 && idx[u] < n4) {
+        pp[u] = ldnt(p4 + idx[u]);
+        gg[u] = ldnt(g4 + idx[u]);
+        mm[u] = ldnt(m4 + idx[u]);
+        vv[u] = ldnt(v4 + idx[u]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t i = idx[u];
+      if (i >= 0 && i < n4) {
+        const float4 ll = perlr ? l4[i] : make_float4(1.f, 1.f, 1.f, 1.f);
+        adam_elem(pp[u].x, gg[u].x, mm[u].x, vv[u].x, ll.x, c);
+        adam_elem(pp[u].y, gg[u].y, mm[u].y, vv[u].y, ll.y, c);
+        adam_elem(pp[u].z, gg[u].z, mm[u].z, vv[u].z, ll.z, c);
+        adam_elem(pp[u].w, gg[u].w, mm[u].w, vv[u].w, ll.w, c);
+        stnt(p4 + i, pp[u]);
+        stnt(m4 + i, mm[u]);
+        stnt(v4 + i, vv[u]);
+        stnt(g4 + i, make_float4(0.f, 0.f, 0.f, 0.f));
+      } else if (i == n4) {
+        for (int64_t e = n4 << 2; e < n; ++e) {
+          float a = p[e], b = m[e], d = v[e];
+          adam_elem(a, g[e], b, d, perlr ? perlr[e] : 1.f, c);
+          p[e] = a;
+          m[e] = b;
+          v[e] = d;
+          g[e] = 0.f;
+        }
       }
     }
   }
@@ -364,7 +380,7 @@ int vx_adam_step_split(int phase, float* param, float* grad, float* exp_avg, flo
     k_adam_dead_pages<U><<<(unsigned)blocks, 256, 0, st>>>(param, exp_avg, exp_avg_sq, per_lr, n, c,
                                                           pages, np);
   } else {
-    const int64_t blocks = std::min<int64_t>(4 * (int64_t)cus, np);
+    const int64_t blocks = std::min<int64_t>(4 * (int64_t)cus, cdiv(np, 4));
     k_adam_live_pages<<<(unsigned)blocks, 256, 0, st>>>(param, grad, exp_avg, exp_avg_sq, per_lr, n,
                                                         c, pages, np);
     // reset the flags and the list count for the next step

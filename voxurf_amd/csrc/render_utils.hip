@@ -177,6 +177,32 @@ __global__ void k_scan_add(int64_t* __restrict__ out, int64_t n,
   if (i < n) out[i] += tile_off[i / kScanTile];
 }
 
+// Single block for short inputs: the tiles in order with a running carry (one launch
+// instead of three).
+__global__ void __launch_bounds__(kScanThreads) k_scan_small(const int64_t* __restrict__ in,
+                                                             int64_t* __restrict__ out, int64_t n,
+                                                             int64_t* __restrict__ total_out) {
+  __shared__ int64_t lds[kScanThreads / 64];
+  int64_t carry = 0;
+  for (int64_t t0 = 0; t0 < n; t0 += kScanTile) {
+    const int64_t base = t0 + threadIdx.x * 4;
+    int64_t v[4];
+    int64_t s = 0;
+    for (int i = 0; i < 4; ++i) {
+      v[i] = (base + i < n) ? in[base + i] : 0;
+      s += v[i];
+    }
+    int64_t total;
+    int64_t off = carry + block_excl_scan(s, lds, total);
+    for (int i = 0; i < 4; ++i) {
+      off += v[i];
+      if (base + i < n) out[base + i] = off;
+    }
+    carry += total;
+  }
+  if (threadIdx.x == 0 && total_out) *total_out = carry;
+}
+
 template <typename T>
 __global__ void k_fill(T* p, int64_t n, T v);
 
@@ -187,6 +213,10 @@ static int scan_i64(const int64_t* in, int64_t* out, int64_t n, int64_t* ws, int
     return launch_status("scan");
   }
   const int64_t ntiles = cdiv(n, kScanTile);
+  if (ntiles <= 16) {
+    k_scan_small<<<1, kScanThreads, 0, st>>>(in, out, n, total);
+    return launch_status("scan");
+  }
   k_scan_tiles<<<(unsigned)ntiles, kScanThreads, 0, st>>>(in, out, n, ws);
   k_scan_sums<<<1, kScanThreads, 0, st>>>(ws, ntiles, total);
   if (ntiles > 1) k_scan_add<<<(unsigned)cdiv(n, 256), 256, 0, st>>>(out, n, ws);

@@ -342,30 +342,83 @@ __device__ __forceinline__ void sdf7(const T* __restrict__ grid, const GridDesc&
   }
 }
 
+// Trilinear cell of the single-channel grid at the zyx index displaced along `axis`
+// (clamped), as sample_displaced / scatter_displaced compute it.
+template <typename T>
+__device__ __forceinline__ void displaced_cell(const GridDesc& g, const T ind[3], int axis, T off,
+                                               Tri<T>& t) {
+#pragma clang fp contract(off)
+  const int sz[3] = {g.Z, g.Y, g.X};
+  T pos[3];
+  for (int k = 0; k < 3; ++k) {
+    T a = ind[k] + (k == axis ? off : (T)0);
+    a = clampT(a, (T)0, (T)(sz[k] - 1));
+    pos[k] = roundtrip_index(a, sz[k]);
+  }
+  t.init(pos[2], pos[1], pos[0]);
+}
+
 // Adjoint of sdf7: ggrid += d(sdf, grad, feat)^T (gsdf, ggrad, gfeat); gfeat may be null.
+// The seven trilinear scatters (centre, then -/+ one voxel along z, y, x of the zyx
+// index) add exactly the reference's per-corner terms; where a displaced cell is the
+// centre cell shifted by one voxel (the interior case), the four corners it shares with
+// the centre are summed in registers first, so a sample issues 32 atomic adds instead of
+// 56.
 template <typename T>
 __device__ __forceinline__ void sdf7_scatter(T* __restrict__ ggrid, const GridDesc& g,
                                              const T q[3], const T* mn, const T* mx,
                                              T voxel_size, T gsdf, const T ggrad[3],
                                              const T* gfeat) {
 #pragma clang fp contract(off)
-  if (gsdf != (T)0) {
-    Tri<T> t;
-    t.init(world_to_index(q[0], mn[0], mx[0] - mn[0], g.X),
-           world_to_index(q[1], mn[1], mx[1] - mn[1], g.Y),
-           world_to_index(q[2], mn[2], mx[2] - mn[2], g.Z));
-    tri_scatter(ggrid, g, t, 0, gsdf);
-  }
+  Tri<T> t0;
+  t0.init(world_to_index(q[0], mn[0], mx[0] - mn[0], g.X),
+          world_to_index(q[1], mn[1], mx[1] - mn[1], g.Y),
+          world_to_index(q[2], mn[2], mx[2] - mn[2], g.Z));
+  T acc[8];
+#pragma unroll
+  for (int b = 0; b < 8; ++b) acc[b] = gsdf != (T)0 ? t0.w(b >> 2, (b >> 1) & 1, b & 1) * gsdf : (T)0;
   T ind[3];
   zyx_index(q, mn, mx, g, ind);
+#pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const int c = 2 - a;
+    const int c = 2 - a;  // world axis of zyx axis a (0 = x)
     const T df = clamped_diff(ind, a, (T)1, g);
     const T gd = (ggrad[c] / df) / voxel_size;
     const T gm = (gfeat ? gfeat[2 * c] : (T)0) - gd;
     const T gp = (gfeat ? gfeat[2 * c + 1] : (T)0) + gd;
-    scatter_displaced(ggrid, g, ind, a, (T)-1, gm);
-    scatter_displaced(ggrid, g, ind, a, (T)1, gp);
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      const int s = side ? 1 : -1;
+      const T go = side ? gp : gm;
+      if (go == (T)0) continue;
+      Tri<T> td;
+      displaced_cell(g, ind, a, (T)s, td);
+      const int d[3] = {td.x0 - t0.x0, td.y0 - t0.y0, td.z0 - t0.z0};
+      const bool shifted = d[c] == s && d[(c + 1) % 3] == 0 && d[(c + 2) % 3] == 0;
+      if (!shifted) {
+        tri_scatter(ggrid, g, td, 0, go);
+        continue;
+      }
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        const int bb[3] = {b >> 2, (b >> 1) & 1, b & 1};
+        const T w = td.w(bb[0], bb[1], bb[2]) * go;
+        const int cb = bb[c] + s;  // this corner's index along axis c in the centre cell
+        if (cb == 0 || cb == 1) {
+          int k[3] = {bb[0], bb[1], bb[2]};
+          k[c] = cb;
+          acc[k[0] * 4 + k[1] * 2 + k[2]] += w;
+        } else {
+          const int x = td.x0 + bb[0], y = td.y0 + bb[1], z = td.z0 + bb[2];
+          if (in_grid<T>(x, y, z, g)) atomic_add(&ggrid[x * g.sx + y * g.sy + z * g.sz], w);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const int x = t0.x0 + (b >> 2), y = t0.y0 + ((b >> 1) & 1), z = t0.z0 + (b & 1);
+    if (acc[b] != (T)0 && in_grid<T>(x, y, z, g)) atomic_add(&ggrid[x * g.sx + y * g.sy + z * g.sz], acc[b]);
   }
 }
 

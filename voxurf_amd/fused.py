@@ -27,8 +27,8 @@ import torch
 
 from . import _lib
 from ._lib import call, ptr, stream_of
+from . import mlp_hip
 from .mlp import RgbNet
-from .optim import Adam
 from .total_variation_cuda import grid_geom
 
 # rgbnet input row width: [k0 (12) | normal (3) | view PE (27)] = 42 features, zero
@@ -71,7 +71,26 @@ class FusedFineTrainer:
         }
         self.rgbnet = RgbNet(cfg.mlp_in, cfg.mlp_width, cfg.mlp_depth, params=mlp_params,
                              compute_dtype=mlp_dtype).to(self.dev)
-        self.mlp_opt = Adam(self.rgbnet.parameters(), lr=cfg.lr_mlp, betas=(0.9, 0.99))
+        mlp_hip.check_shapes(self.rgbnet.linears())
+        if mlp_dtype not in (torch.float32, torch.bfloat16):
+            raise RuntimeError("the fused step runs rgbnet in float32 or bfloat16")
+        # rgbnet parameters (W0, b0, W1, ...) live in one flat buffer, and so do their
+        # gradients and Adam moments: one Adam launch per step for the whole network
+        # (every rgbnet parameter shares one learning rate and step count).
+        params = list(self.rgbnet.parameters())
+        total = sum(p.numel() for p in params)
+        self.mlp_flat = torch.empty(total, dtype=torch.float32, device=self.dev)
+        self.mlp_gflat = torch.zeros(total, dtype=torch.float32, device=self.dev)
+        self.mlp_grads = []
+        off = 0
+        for p in params:
+            k = p.numel()
+            self.mlp_flat[off:off + k].copy_(p.detach().reshape(-1))
+            p.data = self.mlp_flat[off:off + k].view_as(p)
+            self.mlp_grads.append(self.mlp_gflat[off:off + k].view_as(p))
+            off += k
+        self.adam["mlp"] = [torch.zeros_like(self.mlp_flat), torch.zeros_like(self.mlp_flat), 0,
+                            cfg.lr_mlp]
         self.sdf_geom = grid_geom(self.sdf)
         self.k0_geom = grid_geom(self.k0)
         self.box = torch.tensor([-1.0, -1.0, -1.0, 1.0, 1.0, 1.0], dtype=torch.float32)
@@ -115,7 +134,6 @@ class FusedFineTrainer:
         self.glast = torch.empty(n, dtype=torch.float32, device=d)
         self.S = torch.empty(n, dtype=torch.float32, device=d)
         self.loss_r = torch.empty(n, dtype=torch.float32, device=d)
-        self.loss = torch.empty(1, dtype=torch.float32, device=d)
         self._n = n
 
     def _adam(self, name, p, g, pages=None, phase=0, step=None, timed=False):
@@ -185,22 +203,26 @@ class FusedFineTrainer:
                 self._adam("k0", self.k0, self.k0_grad, pages=self.k0_pages, phase=1, step=k0_step,
                            timed=True)
         din = cfg.mlp_in
-        xin = self.X[:m].detach().requires_grad_(True)
-        logits = self.rgbnet.forward_padded(xin)
-        lg = logits.detach().contiguous()
+        lin = self.rgbnet.linears()
+        Ws = [l.weight.detach() for l in lin]
+        bs = [l.bias.detach() for l in lin]
+        bf16 = self.rgbnet.compute_dtype == torch.bfloat16
+        if m > 0:
+            packed = mlp_hip.Packed(Ws, bf16, st)
+            Xr = mlp_hip.padded_rows(self.X[:m], din, bf16)
+            lg, hs = mlp_hip.forward_rows(Xr, Ws, bs, bf16, packed, st)
+        else:
+            lg = torch.zeros((1, 3), dtype=torch.float32, device=self.dev)
         dlogit = torch.empty((max(m, 1), 3), dtype=torch.float32, device=self.dev)
         gw = torch.empty(max(m, 1), dtype=torch.float32, device=self.dev)
         call("vx_fused_fine_composite", n, ptr(self.cnt), ptr(self.off), ptr(lg), 3, ptr(self.P4),
              ptr(self.last), ptr(target), float(cfg.bg), ptr(self.rgb), ptr(dlogit), ptr(gw),
              ptr(self.glast), ptr(self.S), ptr(self.loss_r), st)
-        for p in self.rgbnet.parameters():
-            p.grad = None
         if m > 0:
-            logits.backward(dlogit[:m])
-            dX = xin.grad
-            assert dX.stride(1) == 1
+            dX = mlp_hip.backward_rows(Xr, hs, Ws, bf16, packed, dlogit[:m], st, self.mlp_grads)
         else:
-            dX = torch.zeros((1, din), dtype=torch.float32, device=self.dev)
+            # no survivor: every rgbnet gradient is zero (the buffer is cleared by Adam)
+            dX = torch.zeros((1, XSTRIDE), dtype=torch.float32, device=self.dev)
         dgf = torch.empty((max(m, 1), 3), dtype=torch.float32, device=self.dev)
         call("vx_fused_fine_feat_bwd", self.k0_geom.data_ptr(), self.sdf_geom.data_ptr(),
              self.box.data_ptr(), self.params.data_ptr(), m, ptr(dX), dX.stride(0), ptr(self.P4),
@@ -227,10 +249,12 @@ class FusedFineTrainer:
              ptr(self.glast), ptr(self.S), ptr(dgf), ptr(self.sdf_grad), st)
         if profile:
             self.grads_snapshot = dict(sdf=self.sdf_grad.clone(), k0=k0_grad_snapshot,
-                                       mlp=[p.grad.clone() for p in self.rgbnet.parameters()])
+                                       mlp=[g.clone() for g in self.mlp_grads])
         self._adam("sdf", self.sdf, self.sdf_grad)
-        self.mlp_opt.step()
-        call("vx_sum_f32", ptr(self.loss_r), n, ptr(self.loss), st)
+        self._adam("mlp", self.mlp_flat, self.mlp_gflat)
+        # mean squared error over the batch's N x 3 colour components (run.py:604)
+        loss = torch.empty(1, dtype=torch.float32, device=self.dev)
+        call("vx_sum_f32", ptr(self.loss_r), n, 1.0 / (3.0 * n), ptr(loss), st)
         if self.overlap_adam:
             # the next step's gather reads k0
             main.wait_event(k0_done)
@@ -239,7 +263,7 @@ class FusedFineTrainer:
             s = self.stats[:n].sum(0).tolist()
             self.last_counts = {"m0": int(s[2]), "m0_processed": int(s[0]), "m1b": int(s[1]),
                                 "m2": m}
-        return dict(loss=self.loss / (3.0 * n), rgb_marched=self.rgb[:n], n_samples=m)
+        return dict(loss=loss, rgb_marched=self.rgb[:n], n_samples=m)
 
     def state_dict(self):
         sd = {"sdf.grid": self.sdf, "k0.grid": self.k0}

@@ -6,6 +6,9 @@ every Linear's weight and bias.  Shapes: width 192, at least one hidden layer of
 reference's fine rgbnet 79).  compute_dtype float32 runs FP32 MFMAs; bfloat16 runs the
 forward and data-gradient GEMMs as BF16 MFMAs with FP32 accumulation, stores the
 activations in BF16 and keeps weights, biases, gradients and the optimiser in FP32.
+
+The pieces are also exposed without autograd (pack_net, forward_rows, backward_rows)
+for callers that keep the gradients in their own buffers (the fused training step).
 """
 import torch
 
@@ -21,12 +24,95 @@ def _pad_width(d):
     raise RuntimeError(f"rgbnet input width {d} > 96 is not supported by the MFMA kernels")
 
 
-def _pack(W, kt, nto, transpose, bf16, st):
-    n_out, k_in = W.shape
-    n = nto * kt * 16 * 64
-    out = torch.empty(n, dtype=torch.bfloat16 if bf16 else torch.float32, device=W.device)
-    call("vx_mlp_pack", int(bf16), ptr(W), n_out, k_in, kt, nto, int(transpose), ptr(out), st)
-    return out
+class Packed:
+    """Every MFMA operand of one set of weights (one vx_mlp_pack_net launch):
+    fwd[li] (layer li, forward) and bwd[li] (layer li transposed, backward)."""
+
+    def __init__(self, Ws, bf16, st):
+        n_hidden = len(Ws) - 1
+        k_in0 = Ws[0].shape[1]
+        total = lib().vx_mlp_pack_net_elems(n_hidden, k_in0)
+        if total < 0:
+            raise RuntimeError("rgbnet shape not supported by the MFMA kernels")
+        dev = Ws[0].device
+        self.buf = torch.empty(total, dtype=torch.bfloat16 if bf16 else torch.float32, device=dev)
+        w_ptrs = torch.tensor([W.data_ptr() for W in Ws[:-1]], dtype=torch.int64)
+        call("vx_mlp_pack_net", int(bf16), n_hidden, w_ptrs.data_ptr(), k_in0, ptr(self.buf), st)
+        kt0 = _pad_width(k_in0) // 32
+        self.fwd, self.bwd = [], []
+        off = 0
+        for li in range(n_hidden):
+            kt = kt0 if li == 0 else 6
+            n = lib().vx_mlp_pack_elems(kt, 6)
+            self.fwd.append(self.buf[off:off + n])
+            off += n
+            self.bwd.append(self.buf[off:off + n])
+            off += n
+        self.kt0 = kt0
+
+
+def forward_rows(X, Ws, bs, bf16, packed, st):
+    """logits [M, 3] and the hidden activations of X [M, 32*kt0] (padded rows in the
+    compute element type)."""
+    M = X.shape[0]
+    dev = X.device
+    adt = torch.bfloat16 if bf16 else torch.float32
+    logits = torch.empty((M, 3), dtype=torch.float32, device=dev)
+    hs = []
+    h_in, kt = X, packed.kt0
+    nlin = len(Ws)
+    for li in range(nlin - 1):
+        last = li == nlin - 2
+        H = torch.empty((M, WIDTH), dtype=adt, device=dev)
+        call("vx_mlp_layer_fwd", int(bf16), kt, int(last), ptr(h_in), h_in.shape[1], M,
+             ptr(packed.fwd[li]), ptr(bs[li]), ptr(H), WIDTH, ptr(Ws[-1]) if last else None,
+             ptr(bs[-1]) if last else None, ptr(logits) if last else None, st)
+        hs.append(H)
+        h_in, kt = H, 6
+    return logits, hs
+
+
+def backward_rows(X, hs, Ws, bf16, packed, g, st, grads):
+    """Gradients of every weight and bias into the tensors of `grads` ([W0, b0, W1, ...],
+    overwritten) and the gradient of X [M, 32*kt0] (returned, float32)."""
+    M = X.shape[0]
+    dev = X.device
+    nlin = len(Ws)
+    part = torch.empty(lib().vx_mlp_wgrad_part_elems(M), dtype=torch.float32, device=dev)
+    partb = torch.empty(lib().vx_mlp_wgrad_partb_elems(M), dtype=torch.float32, device=dev)
+    # output layer (K = 3) and the gradient of the last hidden activations
+    dH = torch.empty((M, WIDTH), dtype=torch.float32, device=dev)
+    call("vx_mlp_last_bwd", int(bf16), ptr(g), ptr(hs[-1]), WIDTH, M, ptr(Ws[-1]), ptr(dH),
+         WIDTH, ptr(part), ptr(partb), ptr(grads[2 * (nlin - 1)]), ptr(grads[2 * nlin - 1]), st)
+    dX = None
+    for li in range(nlin - 2, -1, -1):
+        h_prev = hs[li - 1] if li > 0 else X
+        kt = 6 if li > 0 else packed.kt0
+        # weight gradient of layer li: dY = dH (its output grad), X = its input
+        call("vx_mlp_wgrad", int(bf16), kt, Ws[li].shape[1], ptr(dH), WIDTH, ptr(h_prev),
+             h_prev.shape[1], M, ptr(part), ptr(partb), ptr(grads[2 * li]),
+             ptr(grads[2 * li + 1]), st)
+        # data gradient through layer li
+        if li > 0:
+            dprev = torch.empty((M, WIDTH), dtype=torch.float32, device=dev)
+            call("vx_mlp_layer_bwd", int(bf16), kt, ptr(dH), WIDTH, M, ptr(packed.bwd[li]),
+                 ptr(h_prev), WIDTH, ptr(dprev), WIDTH, st)
+            dH = dprev
+        else:
+            dX = torch.empty((M, 32 * kt), dtype=torch.float32, device=dev)
+            call("vx_mlp_layer_bwd", int(bf16), kt, ptr(dH), WIDTH, M, ptr(packed.bwd[0]), None,
+                 0, ptr(dX), 32 * kt, st)
+    return dX
+
+
+def padded_rows(x, k_in, bf16):
+    """x [M, d] -> contiguous [M, 32*kt0] rows in the compute element type (x is used in
+    place when it already has that width; columns past k_in meet zero weights)."""
+    kin = _pad_width(k_in)
+    adt = torch.bfloat16 if bf16 else torch.float32
+    if x.shape[1] == kin and x.is_contiguous():
+        return x.detach().to(adt)
+    return torch.nn.functional.pad(x.detach(), (0, kin - x.shape[1])).to(adt).contiguous()
 
 
 class _MlpFn(torch.autograd.Function):
@@ -36,81 +122,31 @@ class _MlpFn(torch.autograd.Function):
         bs = [p.detach().contiguous() for p in params[1::2]]
         if any(w.dtype != torch.float32 for w in Ws):
             raise RuntimeError("rgbnet parameters must be float32")
-        M, din = x.shape
-        dev = x.device
         st = stream_of(x)
-        kin = _pad_width(Ws[0].shape[1])
-        kt0 = kin // 32
-        adt = torch.bfloat16 if bf16 else torch.float32
-        if din == kin and x.is_contiguous():
-            # already padded to the tile width (columns past in_features meet zero
-            # weights in the packed operand)
-            X = x.detach().to(adt)
-        else:
-            X = torch.nn.functional.pad(x.detach(), (0, kin - din)).to(adt).contiguous()
-        hs = []
-        h_in, kt = X, kt0
-        nlin = len(Ws)
-        logits = torch.empty((M, 3), dtype=torch.float32, device=dev)
-        for li in range(nlin - 1):
-            last = li == nlin - 2
-            pk = _pack(Ws[li], kt, 6, False, bf16, st)
-            H = torch.empty((M, WIDTH), dtype=adt, device=dev)
-            call("vx_mlp_layer_fwd", int(bf16), kt, int(last), ptr(h_in), h_in.shape[1], M, ptr(pk),
-                 ptr(bs[li]), ptr(H), WIDTH, ptr(Ws[-1]) if last else None,
-                 ptr(bs[-1]) if last else None, ptr(logits) if last else None, st)
-            hs.append(H)
-            h_in, kt = H, 6
+        X = padded_rows(x, Ws[0].shape[1], bf16)
+        packed = Packed(Ws, bf16, st)
+        logits, hs = forward_rows(X, Ws, bs, bf16, packed, st)
         ctx.save_for_backward(X, *hs)
-        ctx.Ws = Ws
-        ctx.bf16 = bf16
-        ctx.din = din
-        ctx.kt0 = kt0
+        ctx.Ws, ctx.bs, ctx.packed, ctx.bf16, ctx.din = Ws, bs, packed, bf16, x.shape[1]
         return logits
 
     @staticmethod
     @torch.autograd.function.once_differentiable
     def backward(ctx, glogits):
         X, *hs = ctx.saved_tensors
-        Ws, bf16, din, kt0 = ctx.Ws, ctx.bf16, ctx.din, ctx.kt0
-        M = X.shape[0]
-        dev = X.device
-        st = stream_of(X)
-        g = glogits.contiguous().float()
-        nlin = len(Ws)
-        grads = [None] * (2 * nlin)
-        part = torch.empty(lib().vx_mlp_wgrad_part_elems(M), dtype=torch.float32, device=dev)
-        partb = torch.empty(lib().vx_mlp_wgrad_partb_elems(M), dtype=torch.float32, device=dev)
-        # output layer (K = 3) and the gradient of the last hidden activations
-        dW = torch.empty_like(Ws[-1])
-        db = torch.empty(3, dtype=torch.float32, device=dev)
-        dH = torch.empty((M, WIDTH), dtype=torch.float32, device=dev)
-        call("vx_mlp_last_bwd", int(bf16), ptr(g), ptr(hs[-1]), WIDTH, M, ptr(Ws[-1]), ptr(dH),
-             WIDTH, ptr(part), ptr(partb), ptr(dW), ptr(db), st)
-        grads[2 * (nlin - 1)], grads[2 * (nlin - 1) + 1] = dW, db
-        dX = None
-        for li in range(nlin - 2, -1, -1):
-            h_prev = hs[li - 1] if li > 0 else X
-            kt = 6 if li > 0 else kt0
-            # weight gradient of layer li: dY = dH (its output grad), X = its input
-            dW = torch.empty_like(Ws[li])
-            db = torch.empty(WIDTH, dtype=torch.float32, device=dev)
-            call("vx_mlp_wgrad", int(bf16), kt, Ws[li].shape[1], ptr(dH), WIDTH, ptr(h_prev),
-                 h_prev.shape[1], M, ptr(part), ptr(partb), ptr(dW), ptr(db), st)
-            grads[2 * li], grads[2 * li + 1] = dW, db
-            # data gradient through layer li
-            pkT = _pack(Ws[li], 6, kt, True, bf16, st)
-            if li > 0:
-                dprev = torch.empty((M, WIDTH), dtype=torch.float32, device=dev)
-                call("vx_mlp_layer_bwd", int(bf16), kt, ptr(dH), WIDTH, M, ptr(pkT), ptr(h_prev),
-                     WIDTH, ptr(dprev), WIDTH, st)
-                dH = dprev
-            else:
-                dXp = torch.empty((M, 32 * kt), dtype=torch.float32, device=dev)
-                call("vx_mlp_layer_bwd", int(bf16), kt, ptr(dH), WIDTH, M, ptr(pkT), None, 0,
-                     ptr(dXp), 32 * kt, st)
-                dX = dXp[:, :din]
-        return (dX, None, *grads)
+        Ws, bs = ctx.Ws, ctx.bs
+        grads = []
+        for W, b in zip(Ws, bs):
+            grads += [torch.empty_like(W), torch.empty_like(b)]
+        dX = backward_rows(X, hs, Ws, ctx.bf16, ctx.packed, glogits.contiguous().float(),
+                           stream_of(X), grads)
+        return (dX[:, :ctx.din], None, *grads)
+
+
+def check_shapes(linears):
+    if len(linears) < 3 or any(l.out_features != WIDTH for l in linears[:-1]) or \
+            linears[-1].out_features != 3 or any(l.in_features != WIDTH for l in linears[1:]):
+        raise RuntimeError("rgbnet kernels support width 192, depth >= 3, 3 outputs")
 
 
 def mlp_forward(x, linears, compute_dtype=torch.float32, padded=False):
@@ -121,9 +157,7 @@ def mlp_forward(x, linears, compute_dtype=torch.float32, padded=False):
     are read in place and the gradient has the same [M, W] shape."""
     if not x.is_cuda:
         raise RuntimeError("the rgbnet kernels need CUDA (HIP) tensors")
-    if len(linears) < 3 or any(l.out_features != WIDTH for l in linears[:-1]) or \
-            linears[-1].out_features != 3 or any(l.in_features != WIDTH for l in linears[1:]):
-        raise RuntimeError("rgbnet kernels support width 192, depth >= 3, 3 outputs")
+    check_shapes(linears)
     bf16 = compute_dtype == torch.bfloat16
     params = []
     for l in linears:
