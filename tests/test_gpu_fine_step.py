@@ -126,3 +126,33 @@ def test_fused_equals_ops_survivors_exactly(dev, setup):
     assert torch.equal(f.rid[: f.m].long(), out["ray_id"])
     assert torch.equal(f.P4[: f.m, 3], out["weights"])
     assert torch.equal(f.last, out["alphainv_cum"])
+
+
+def test_split_k0_update_over_consecutive_steps(dev, setup):
+    """Three consecutive fused steps with the k0 Adam pass split by page flags and run
+    on the second stream (alternating flag buffers, page lists built by phase 1): after
+    every step, k0 and its Adam moments equal utils.Adam applied, in order and to the
+    whole grid, to the state before the step and that step's k0 gradient.  (Comparing
+    two multi-step runs directly is ill-posed: the gradient atomics perturb the weights
+    by rounding, a sample whose weight sits at the 1e-4 threshold can flip, and Adam
+    turns a flipped 1e-10 gradient into a full-size step on the 8 corners involved.)"""
+    st, (o, d, v, tg), _, _ = setup
+    O, D, V, TG = (x.to(dev) for x in (o, d, v, tg))
+    t = FusedFineTrainer(CFG, st["sdf"], st["k0"], st["mlp"], device=dev)
+    assert t.overlap_adam
+    p = t.k0.detach().cpu().double()
+    m = torch.zeros_like(p)
+    vv = torch.zeros_like(p)
+    for i in range(3):
+        k = (i * 97) % CFG.n_rays  # a different batch each step: rotate the rays
+        t.step(torch.roll(O, k, 0), torch.roll(D, k, 0), torch.roll(V, k, 0),
+               torch.roll(TG, k, 0), profile=True)
+        g = t.grads_snapshot["k0"].cpu().double()
+        tr.adam_step(p, g, m, vv, i + 1, CFG.lr_k0, 0.9, 0.99, 1e-8)
+        close(t.k0, p, 1e-5, 1e-7, f"k0 after step {i + 1}")
+        close(t.adam["k0"][0], m, 1e-5, 1e-7, f"k0 exp_avg after step {i + 1}")
+        close(t.adam["k0"][1], vv, 1e-5, 1e-7, f"k0 exp_avg_sq after step {i + 1}")
+        # the next step starts from the trainer's own state (removes float32 drift)
+        p = t.k0.detach().cpu().double()
+        m = t.adam["k0"][0].detach().cpu().double()
+        vv = t.adam["k0"][1].detach().cpu().double()

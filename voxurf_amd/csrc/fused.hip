@@ -106,6 +106,26 @@ __device__ __forceinline__ void run_chunk(const FusedGeom& G, const float* start
   cs.surv = cs.inrange && cs.w > G.thres;
 }
 
+__device__ __forceinline__ void mark_page(uint32_t* pages, int64_t addr) {
+  pages[addr >> 10] = 1u;
+}
+
+// Flag the page(s) holding the C channels of trilinear corner `corner` (0..7, x outer,
+// z inner) of t: one run of C floats when channels are innermost, C strided words
+// otherwise.
+__device__ __forceinline__ void mark_corner_pages(uint32_t* pages, const GridDesc& g,
+                                                  const Tri<float>& t, int corner) {
+  const int x = t.x0 + (corner >> 2), y = t.y0 + ((corner >> 1) & 1), z = t.z0 + (corner & 1);
+  if (!in_grid<float>(x, y, z, g)) return;
+  const int64_t o = x * g.sx + y * g.sy + z * g.sz;
+  if (g.sc == 1) {
+    mark_page(pages, o);
+    mark_page(pages, o + g.C - 1);
+  } else {
+    for (int ch = 0; ch < g.C; ++ch) mark_page(pages, o + ch * g.sc);
+  }
+}
+
 // Alpha and weight of a stored walk-range sample, recomputed with the forward's own
 // arithmetic (bit-identical to the walk's values).
 struct SlotSample {
@@ -160,7 +180,8 @@ __global__ void __launch_bounds__(256) k_fused_fwd(
     FusedGeom G, const float* __restrict__ rays_o, const float* __restrict__ rays_d,
     const float* __restrict__ viewdirs, int64_t n_rays, int cap, float4* __restrict__ s4,
     float4* __restrict__ g4, int64_t* __restrict__ cnt_out, float* __restrict__ last_out,
-    int* __restrict__ stats, int count_all, int* __restrict__ overflow) {
+    int* __restrict__ stats, int count_all, int* __restrict__ overflow,
+    uint32_t* __restrict__ k0_pages) {
   const int lane = threadIdx.x & 63;
   const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
   for (int64_t r = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6); r < n_rays;
@@ -188,6 +209,14 @@ __global__ void __launch_bounds__(256) k_fused_fwd(
       }
       m1 += __popcll(im);
       cnt += __popcll(__ballot(cs.surv));
+      if (k0_pages && cs.surv) {
+        // every k0_grad page the feature scatter of this survivor can write
+        Tri<float> t;
+        t.init(world_to_index(cs.q[0], G.mn[0], G.mx[0] - G.mn[0], G.gk.X),
+               world_to_index(cs.q[1], G.mn[1], G.mx[1] - G.mn[1], G.gk.Y),
+               world_to_index(cs.q[2], G.mn[2], G.mx[2] - G.mn[2], G.gk.Z));
+        for (int corner = 0; corner < 8; ++corner) mark_corner_pages(k0_pages, G.gk, t, corner);
+      }
     }
     int m0_all = m0;
     if (count_all) {
@@ -211,26 +240,6 @@ __global__ void __launch_bounds__(256) k_fused_fwd(
       stats[3 * r + 1] = m1;
       stats[3 * r + 2] = m0_all;
     }
-  }
-}
-
-__device__ __forceinline__ void mark_page(uint32_t* pages, int64_t addr) {
-  pages[addr >> 10] = 1u;
-}
-
-// Flag the page(s) holding the C channels of trilinear corner `corner` (0..7, x outer,
-// z inner) of t: one run of C floats when channels are innermost, C strided words
-// otherwise.
-__device__ __forceinline__ void mark_corner_pages(uint32_t* pages, const GridDesc& g,
-                                                  const Tri<float>& t, int corner) {
-  const int x = t.x0 + (corner >> 2), y = t.y0 + ((corner >> 1) & 1), z = t.z0 + (corner & 1);
-  if (!in_grid<float>(x, y, z, g)) return;
-  const int64_t o = x * g.sx + y * g.sy + z * g.sz;
-  if (g.sc == 1) {
-    mark_page(pages, o);
-    mark_page(pages, o + g.C - 1);
-  } else {
-    for (int ch = 0; ch < g.C; ++ch) mark_page(pages, o + ch * g.sc);
   }
 }
 
@@ -613,15 +622,16 @@ int vx_fused_fine_fwd(const float* sdf, const int64_t* sdf_geom, const float* bo
                       const float* mc_params, const float* rays_o, const float* rays_d,
                       const float* viewdirs, int64_t n_rays, int cap, void* s4, void* g4,
                       int64_t* cnt, float* last, int* stats, int count_all, int* overflow,
-                      void* stream) {
+                      const int64_t* k0_geom, uint32_t* k0_pages, void* stream) {
   FusedGeom G;
-  if (int st = fused_geom(G, sdf, sdf_geom, nullptr, nullptr, box, params)) return st;
+  VX_CHECK_ARG(!k0_pages || k0_geom, "fused_fine_fwd: k0_pages needs k0_geom");
+  if (int st = fused_geom(G, sdf, sdf_geom, nullptr, k0_geom, box, params)) return st;
   if (int st = fused_maskcache(G, mc, mc_geom, mc_params)) return st;
   VX_CHECK_ARG(n_rays >= 0 && cap >= 1, "fused_fine_fwd: bad sizes");
   if (n_rays == 0) return VX_OK;
   k_fused_fwd<<<grid_for(n_rays, 4, 1 << 16), 256, 0, (hipStream_t)stream>>>(
       G, rays_o, rays_d, viewdirs, n_rays, cap, (float4*)s4, (float4*)g4, cnt, last, stats,
-      count_all, overflow);
+      count_all, overflow, k0_pages);
   return launch_status("fused_fine_fwd");
 }
 

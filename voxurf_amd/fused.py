@@ -4,7 +4,8 @@ One step (run.py:600-659 for the fine model, lib/voxurf_fine.py:620-802):
 
   1. vx_fused_fine_fwd     one wavefront per ray: sampling, SDF + 6-neighbour gradient,
                            NeuS alpha, the transmittance walk; the walk-range samples
-                           (the only ones with a gradient) into per-ray slots
+                           (the only ones with a gradient) into per-ray slots; flags the
+                           k0 pages the survivors' feature scatter can reach
   2. scan of survivor counts -> row offsets; the total is read back once (the
                            reference reads sample totals back several times per step)
   3. vx_fused_fine_gather  survivors (w > thres) + rgbnet input rows [k0 | normal | view PE]
@@ -14,9 +15,12 @@ One step (run.py:600-659 for the fine model, lib/voxurf_fine.py:620-802):
   7. vx_fused_fine_feat_bwd   k0 scatter, normal adjoint
   8. vx_fused_fine_bwd        per-ray backward of the SDF side over the stored walk range
   9. Adam (utils.Adam semantics) over sdf, k0 and rgbnet, clearing the grid gradients.
-     The k0 pass is split by the page flags the gather sets: the pages no survivor can
-     reach (all-zero gradient) are updated on a second stream right after step 3,
-     concurrently with steps 4-6; the flagged pages after step 7, concurrently with 8.
+
+Streams.  The k0 Adam pass (the largest HBM sweep of the step) runs on a second stream
+in two halves (vx_adam_step_split): the unflagged pages, whose gradient is zero this
+step, from the end of step 1 -- concurrently with steps 2-8 -- and the flagged pages
+after step 7.  The main stream waits for it only before the next step's gather reads
+k0, so it also overlaps the next forward.
 
 Grids: sdf [1,1,R,R,R] contiguous; k0 [1,C,R,R,R] channels-last.  Gradient buffers
 persist across steps (cleared by the Adam pass instead of reallocated).
@@ -56,18 +60,23 @@ class FusedFineTrainer:
         self.cfg = cfg
         self.dev = torch.device(device)
         self.sdf = sdf.to(self.dev, torch.float32).contiguous()
-        self.k0 = k0.to(self.dev, torch.float32).contiguous(memory_format=torch.channels_last_3d)
+        self._k0 = k0.to(self.dev, torch.float32).contiguous(memory_format=torch.channels_last_3d)
+        self._k0_done = None         # event: the last step's k0 update has finished
         self.sdf_grad = torch.zeros_like(self.sdf)
-        self.k0_grad = torch.zeros_like(self.k0)
-        # one flag per 1024-element page of k0_grad, set by the gather for every page the
-        # feature scatter can write; followed by the count and list of flagged pages that
-        # the split Adam builds (vx_adam_step_split)
-        n_pages = (self.k0.numel() + 1023) // 1024
-        self.k0_pages = torch.zeros(2 * n_pages + 1, dtype=torch.int32, device=self.dev)
+        self.k0_grad = torch.zeros_like(self._k0)
+        # Per step, one flag per 1024-element page of k0_grad, set by the forward for every
+        # page the feature scatter can write, followed by the count and list of flagged
+        # pages the split Adam builds (vx_adam_step_split).  Two sets, alternating by
+        # step, because a step's k0 update may still be running when the next forward
+        # flags its pages.
+        n_pages = (self._k0.numel() + 1023) // 1024
+        self._pages = [torch.zeros(2 * n_pages + 1, dtype=torch.int32, device=self.dev)
+                       for _ in range(2)]
+        self._parity = 0
         self.n_pages = n_pages
         self.adam = {
             "sdf": [torch.zeros_like(self.sdf), torch.zeros_like(self.sdf), 0, cfg.lr_sdf],
-            "k0": [torch.zeros_like(self.k0), torch.zeros_like(self.k0), 0, cfg.lr_k0],
+            "k0": [torch.zeros_like(self._k0), torch.zeros_like(self._k0), 0, cfg.lr_k0],
         }
         self.rgbnet = RgbNet(cfg.mlp_in, cfg.mlp_width, cfg.mlp_depth, params=mlp_params,
                              compute_dtype=mlp_dtype).to(self.dev)
@@ -92,7 +101,7 @@ class FusedFineTrainer:
         self.adam["mlp"] = [torch.zeros_like(self.mlp_flat), torch.zeros_like(self.mlp_flat), 0,
                             cfg.lr_mlp]
         self.sdf_geom = grid_geom(self.sdf)
-        self.k0_geom = grid_geom(self.k0)
+        self.k0_geom = grid_geom(self._k0)
         self.box = torch.tensor([-1.0, -1.0, -1.0, 1.0, 1.0, 1.0], dtype=torch.float32)
         self.params = torch.tensor([cfg.stepdist, cfg.voxel_size, cfg.inv_s, cfg.fast_color_thres,
                                     0.0, 1e9], dtype=torch.float32)
@@ -103,7 +112,8 @@ class FusedFineTrainer:
         self.last_counts = {}
         self.count_all = True
         self.side = torch.cuda.Stream(self.dev)
-        # False: the k0 Adam pass runs in one piece, in order on the main stream
+        # True: the k0 Adam pass runs on the second stream in two phases (see step());
+        # False: in one piece, in order on the main stream
         self.overlap_adam = True
         # a list: (start, end) events of the k0 Adam sweep (phase 1, or the whole pass when
         # not overlapped) of every step are appended to it
@@ -111,6 +121,14 @@ class FusedFineTrainer:
         self.last_live_pages = None
 
     mask_cache = None
+
+    @property
+    def k0(self):
+        """The feature grid, once every pending update of it has finished (for reads on
+        torch's current stream)."""
+        if self._k0_done is not None:
+            torch.cuda.current_stream(self.dev).wait_event(self._k0_done)
+        return self._k0
 
     def _mc_args(self):
         """(density ptr, geom ptr, params ptr) of the optional MaskCache, or NULLs."""
@@ -162,13 +180,30 @@ class FusedFineTrainer:
     # --- one step -----------------------------------------------------------------
     def forward(self, rays_o, rays_d, viewdirs):
         """Steps 1-3: survivors and rgbnet input rows; returns M (survivor count)."""
+        return self._forward(rays_o, rays_d, viewdirs, None, None)
+
+    def _forward(self, rays_o, rays_d, viewdirs, pages, k0_step):
         n = rays_o.shape[0]
         self._ensure(n)
         st = stream_of(self.sdf)
+        main = torch.cuda.current_stream(self.dev)
         call("vx_fused_fine_fwd", ptr(self.sdf), self.sdf_geom.data_ptr(), self.box.data_ptr(),
              self.params.data_ptr(), *self._mc_args(), ptr(rays_o), ptr(rays_d), ptr(viewdirs), n, self.cap,
              ptr(self.s4), ptr(self.g4), ptr(self.cnt), ptr(self.last), ptr(self.stats),
-             int(self.count_all), ptr(self.overflow), st)
+             int(self.count_all), ptr(self.overflow),
+             self.k0_geom.data_ptr() if pages is not None else None, ptr(pages), st)
+        if pages is not None and self.overlap_adam:
+            # The forward has flagged every k0 page the feature scatter can write; the
+            # other pages have an all-zero gradient this step, so their Adam update (most
+            # of the 256^3 x 12 sweep, HBM-bound) starts now on the second stream and runs
+            # beside the gather and the MFMA-bound rgbnet passes.  The gather reads k0
+            # only at survivor corners, i.e. in flagged pages, which this pass leaves alone.
+            flags_ready = torch.cuda.Event()
+            flags_ready.record(main)
+            with torch.cuda.stream(self.side):
+                self.side.wait_event(flags_ready)
+                self._adam("k0", self._k0, self.k0_grad, pages=pages, phase=1, step=k0_step,
+                           timed=True)
         call("vx_inclusive_scan_i64", ptr(self.cnt), ptr(self.off), n, ptr(self.ws), st)
         m = int(self.off[-1].item()) if n > 0 else 0
         self.m = m
@@ -176,32 +211,30 @@ class FusedFineTrainer:
         self.P4 = torch.empty((max(m, 1), 4), dtype=torch.float32, device=self.dev)
         self.G4 = torch.empty((max(m, 1), 4), dtype=torch.float32, device=self.dev)
         self.rid = torch.empty(max(m, 1), dtype=torch.int32, device=self.dev)
-        call("vx_fused_fine_gather", ptr(self.k0), self.k0_geom.data_ptr(),
+        # the previous step's k0 update must have finished before k0 is read
+        if self._k0_done is not None:
+            main.wait_event(self._k0_done)
+            self._k0_done = None
+        call("vx_fused_fine_gather", ptr(self._k0), self.k0_geom.data_ptr(),
              self.sdf_geom.data_ptr(), self.box.data_ptr(), self.params.data_ptr(),
              ptr(viewdirs), n, self.cap, self.cfg.view_pe, ptr(self.s4), ptr(self.g4),
              ptr(self.stats), ptr(self.cnt), ptr(self.off), XSTRIDE, ptr(self.X), ptr(self.P4),
-             ptr(self.G4), ptr(self.rid), ptr(self.k0_pages), st)
+             ptr(self.G4), ptr(self.rid), None, st)
         return m
 
     def step(self, rays_o, rays_d, viewdirs, target, profile=False):
+        """One training step.  With overlap_adam the k0 update may still be running on the
+        second stream when this returns; the next step, the `k0` property and
+        state_dict() wait for it."""
         cfg = self.cfg
         n = rays_o.shape[0]
-        m = self.forward(rays_o, rays_d, viewdirs)
-        st = stream_of(self.sdf)
-        main = torch.cuda.current_stream(self.dev)
+        pages = self._pages[self._parity]
+        self._parity ^= 1
         k0_step = self.adam["k0"][2] + 1
         self.adam["k0"][2] = k0_step
-        if self.overlap_adam:
-            # The gather has flagged every k0 page the feature scatter can write; the
-            # others have an all-zero gradient this step, so their Adam update (most of
-            # the 256^3 x 12 sweep, HBM-bound) runs now on a second stream, beside the
-            # MFMA-bound rgbnet passes.
-            flags_ready = torch.cuda.Event()
-            flags_ready.record(main)
-            with torch.cuda.stream(self.side):
-                self.side.wait_event(flags_ready)
-                self._adam("k0", self.k0, self.k0_grad, pages=self.k0_pages, phase=1, step=k0_step,
-                           timed=True)
+        m = self._forward(rays_o, rays_d, viewdirs, pages, k0_step)
+        st = stream_of(self.sdf)
+        main = torch.cuda.current_stream(self.dev)
         din = cfg.mlp_in
         lin = self.rgbnet.linears()
         Ws = [l.weight.detach() for l in lin]
@@ -230,18 +263,18 @@ class FusedFineTrainer:
         if profile:
             # snapshot before any optimiser pass clears a gradient
             k0_grad_snapshot = self.k0_grad.clone()
-            self.last_live_pages = int(self.k0_pages[:self.n_pages].count_nonzero())
+            self.last_live_pages = int(pages[:self.n_pages].count_nonzero())
         if self.overlap_adam:
             # the flagged pages, once the feature scatter has finished
             grads_ready = torch.cuda.Event()
             grads_ready.record(main)
             with torch.cuda.stream(self.side):
                 self.side.wait_event(grads_ready)
-                self._adam("k0", self.k0, self.k0_grad, pages=self.k0_pages, phase=2, step=k0_step)
-                k0_done = torch.cuda.Event()
-                k0_done.record(self.side)
+                self._adam("k0", self._k0, self.k0_grad, pages=pages, phase=2, step=k0_step)
+                self._k0_done = torch.cuda.Event()
+                self._k0_done.record(self.side)
         else:
-            self._adam("k0", self.k0, self.k0_grad, pages=self.k0_pages, phase=0, step=k0_step,
+            self._adam("k0", self._k0, self.k0_grad, pages=pages, phase=0, step=k0_step,
                        timed=True)
         call("vx_fused_fine_bwd", ptr(self.sdf), self.sdf_geom.data_ptr(), self.box.data_ptr(),
              self.params.data_ptr(), ptr(viewdirs), n, self.cap, ptr(self.s4), ptr(self.g4),
@@ -255,9 +288,6 @@ class FusedFineTrainer:
         # mean squared error over the batch's N x 3 colour components (run.py:604)
         loss = torch.empty(1, dtype=torch.float32, device=self.dev)
         call("vx_sum_f32", ptr(self.loss_r), n, 1.0 / (3.0 * n), ptr(loss), st)
-        if self.overlap_adam:
-            # the next step's gather reads k0
-            main.wait_event(k0_done)
         if profile:
             torch.cuda.synchronize()
             s = self.stats[:n].sum(0).tolist()
