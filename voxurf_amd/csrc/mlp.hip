@@ -125,8 +125,10 @@ __global__ void k_pack_net(PackNet net, void* __restrict__ out) {
 // (MODE 0) in the layer's element type, gradients (MODE 2/3) float32.  Output rows
 // [M, out_stride]: activations in the element type (bfloat16 when BF), gradients
 // float32.  Persistent: each wave walks 32-row tiles; weights stay in LDS.
+constexpr int kLayerThreads = 512;
+
 template <int KT, int NTO, int MODE, bool BF>
-__global__ void __launch_bounds__(512) k_mlp_layer(
+__global__ void __launch_bounds__(kLayerThreads) k_mlp_layer(
     const void* __restrict__ in_, int in_stride, int64_t M, const void* __restrict__ wpk,
     const float* __restrict__ bias, const void* __restrict__ mask_, int mask_stride,
     void* __restrict__ out_, int out_stride) {
@@ -136,10 +138,23 @@ __global__ void __launch_bounds__(512) k_mlp_layer(
   __shared__ float lb[NTO * 32];
   const int tid = threadIdx.x;
   {
+    // weights into LDS: every load of a thread in flight before its first store (a
+    // load-store loop one vector at a time serialises on the L2 latency)
     const float4* src = reinterpret_cast<const float4*>(wpk);
     float4* dst = reinterpret_cast<float4*>(lds_raw);
     constexpr int n4 = WELEMS * (BF ? 2 : 4) / 16;
-    for (int i = tid; i < n4; i += blockDim.x) dst[i] = src[i];
+    constexpr int P = (n4 + kLayerThreads - 1) / kLayerThreads;  // vectors per thread
+    float4 v[P];
+#pragma unroll
+    for (int b = 0; b < P; ++b) {
+      const int i = tid + b * kLayerThreads;
+      v[b] = i < n4 ? src[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int b = 0; b < P; ++b) {
+      const int i = tid + b * kLayerThreads;
+      if (i < n4) dst[i] = v[b];
+    }
     if (MODE == 0)
       for (int i = tid; i < NTO * 32; i += blockDim.x) lb[i] = bias[i];
   }
@@ -587,7 +602,7 @@ static void launch_layer(const void* in, int in_stride, int64_t M, const void* w
   const int64_t cap = num_cus();
   if (blocks > cap) blocks = cap;
   if (blocks < 1) blocks = 1;
-  k_mlp_layer<KT, NTO, MODE, BF><<<(unsigned)blocks, 512, 0, st>>>(
+  k_mlp_layer<KT, NTO, MODE, BF><<<(unsigned)blocks, kLayerThreads, 0, st>>>(
       in, in_stride, M, wpk, bias, mask, mask_stride, out, out_stride);
 }
 
