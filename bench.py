@@ -172,6 +172,13 @@ def main():
         roof_kernel = "adam_step (k0 grid, 256^3 x 12 fp32)"
         adam_bytes = 32 * n_k0
     achieved = adam_bytes / (adam_ms * 1e-3) / 1e9
+    # HBM bytes per launch of that kernel from rocprofv3 FETCH_SIZE / WRITE_SIZE passes
+    # (calibrated on a 1 GiB copy), when a measurement for it is on file
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "adam_phase1_traffic.json")
+    if args.path == "fused" and trainer.overlap_adam and os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f)["hbm_bytes_per_launch"]
     counts = trainer.last_counts
     sb = step_bytes(cfg, counts)
 
@@ -196,7 +203,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": roof_kernel, "ms": adam_ms,
                      "algorithmic_bytes": adam_bytes,
                      "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": None},
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic},
         "step_roofline": None if sb is None else {
             "algorithmic_bytes": sb, "achieved_GBs": sb / (ms * 1e-3) / 1e9,
             "frac": sb / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS},
