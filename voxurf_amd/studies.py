@@ -12,6 +12,7 @@ serve      the continuous-batching server vs sequential per-request rendering on
 """
 import dataclasses
 import math
+import sys
 import time
 
 import torch
@@ -204,6 +205,7 @@ def precision(args):
             }
             del t
         rows.append(row)
+        print(f"precision: seed {seed} done", file=sys.stderr, flush=True)
         del ref
         torch.cuda.empty_cache()
     summary = {}
@@ -232,6 +234,9 @@ def precision(args):
         t = FusedFineTrainer(cfg, sdf, k0, mlp, device=dev, mlp_dtype=dt)
         for i in range(args.psnr_steps):
             t.step(*batches[i % n_pool])
+            if (i + 1) % 50 == 0:
+                torch.cuda.synchronize()
+                print(f"precision: {name} step {i + 1}/{args.psnr_steps}", file=sys.stderr, flush=True)
         o, d, v, tg = held
         t.forward(o, d, v)
         pred = _render_with(t, o, d, v)
