@@ -183,6 +183,40 @@ int vx_maskcache_alpha(int dtype, const void* density, const int64_t* geom, cons
                        double voxel_size_ratio, double thres, int nearest, int64_t n,
                        uint8_t* out, void* stream);
 
+/* ---- camera rays and the MaskCache ray pre-filter --------------------------------
+ * cam: 26 floats in host memory, {H, W, K[0][0], K[1][1], K[0][2], K[1][2], c2w[:3,:3]
+ * row-major (9), c2w[:3,3] (3), pixel offset (0.5 'center' / 0 'lefttop'), inverse_y,
+ * flip_x, flip_y, ndc, -1/(W/(2 focal)), -1/(H/(2 focal)), ndc near}. */
+
+/* lib/voxurf_fine.py:1062-1068 get_rays_of_a_view (with get_rays :1001-1029 and
+ * ndc_rays :1042-1059): rays_o, rays_d, viewdirs [H*W, 3] float32, row-major pixels. */
+int vx_camera_rays(const float* cam, float* rays_o, float* rays_d, float* viewdirs, void* stream);
+
+/* lib/voxurf_fine.py:1042-1059 ndc_rays on n rays; c0 = -1/(W/(2 focal)),
+ * c1 = -1/(H/(2 focal)). */
+int vx_ndc_rays(int64_t n, float c0, float c1, float near_, const float* rays_o,
+                const float* rays_d, float* o_out, float* d_out, void* stream);
+
+/* lib/voxurf_fine.py:579-591 Voxurf.hit_coarse_geo: does any in-box sample of the ray
+ * (sample_pts_on_rays with far = 1e9) pass the MaskCache?  box = {xyz_min, xyz_max},
+ * params = {stepdist, near} (host); mc / mc_geom / mc_params as vx_fused_fine_fwd.
+ * Outputs: hit [n] bytes and/or flag [n] int64 (either may be NULL). */
+int vx_hit_coarse_geo(const float* rays_o, const float* rays_d, int64_t n_rays, const float* box,
+                      const float* params, const float* mc, const int64_t* mc_geom,
+                      const float* mc_params, uint8_t* hit, int64_t* flag, void* stream);
+
+/* hit_coarse_geo on the rays of one view (lib/voxurf_fine.py:1145-1149), the rays made
+ * in the kernel: flag [H*W] int64. */
+int vx_view_hit_coarse_geo(const float* cam, const float* box, const float* params,
+                           const float* mc, const int64_t* mc_geom, const float* mc_params,
+                           int64_t* flag, void* stream);
+
+/* lib/voxurf_fine.py:1150-1155 (img[mask], rays_o[mask], ...): incl = inclusive scan of
+ * the view's flags; kept pixels are written in pixel order to rows incl[p] - 1 of
+ * rgb_out, o_out, d_out, v_out [*, 3] (rgb [H*W, 3] float32). */
+int vx_view_compact_rays(const float* cam, const float* rgb, const int64_t* incl, float* rgb_out,
+                         float* o_out, float* d_out, float* v_out, void* stream);
+
 /* Zero n elements (a fill kernel, safe inside captured graphs). */
 int vx_zero(int dtype, void* p, int64_t n, void* stream);
 

@@ -155,6 +155,79 @@ def maskcache_alpha(density, xyz, xyz_min, xyz_max, act_shift, voxel_size_ratio,
     return alpha.reshape(*shape)
 
 
+# ---- camera rays and the MaskCache ray pre-filter --------------------------------------
+
+def get_rays(H, W, K, c2w, inverse_y, flip_x, flip_y, mode="center"):
+    """lib/voxurf_fine.py:1001-1029 (mode 'center' or 'lefttop'); K and c2w as given
+    (numpy or tensor), float32 rays."""
+    i, j = torch.meshgrid(torch.linspace(0, W - 1, W), torch.linspace(0, H - 1, H), indexing="ij")
+    i = i.t().float()
+    j = j.t().float()
+    if mode == "center":
+        i, j = i + 0.5, j + 0.5
+    elif mode != "lefttop":
+        raise NotImplementedError(mode)
+    if flip_x:
+        i = i.flip((1,))
+    if flip_y:
+        j = j.flip((0,))
+    if inverse_y:
+        dirs = torch.stack([(i - K[0][2]) / K[0][0], (j - K[1][2]) / K[1][1], torch.ones_like(i)], -1)
+    else:
+        dirs = torch.stack([(i - K[0][2]) / K[0][0], -(j - K[1][2]) / K[1][1], -torch.ones_like(i)], -1)
+    c2w = torch.as_tensor(c2w, dtype=torch.float32)
+    rays_d = torch.sum(dirs[..., None, :] * c2w[:3, :3], -1)
+    rays_o = c2w[:3, 3].expand(rays_d.shape)
+    return rays_o, rays_d
+
+
+def ndc_rays(H, W, focal, near, rays_o, rays_d):
+    """lib/voxurf_fine.py:1042-1059."""
+    t = -(near + rays_o[..., 2]) / rays_d[..., 2]
+    rays_o = rays_o + t[..., None] * rays_d
+    o0 = -1. / (W / (2. * focal)) * rays_o[..., 0] / rays_o[..., 2]
+    o1 = -1. / (H / (2. * focal)) * rays_o[..., 1] / rays_o[..., 2]
+    o2 = 1. + 2. * near / rays_o[..., 2]
+    d0 = -1. / (W / (2. * focal)) * (rays_d[..., 0] / rays_d[..., 2] - rays_o[..., 0] / rays_o[..., 2])
+    d1 = -1. / (H / (2. * focal)) * (rays_d[..., 1] / rays_d[..., 2] - rays_o[..., 1] / rays_o[..., 2])
+    d2 = -2. * near / rays_o[..., 2]
+    return torch.stack([o0, o1, o2], -1), torch.stack([d0, d1, d2], -1)
+
+
+def get_rays_of_a_view(H, W, K, c2w, ndc, inverse_y, flip_x, flip_y, mode="center"):
+    """lib/voxurf_fine.py:1062-1068."""
+    rays_o, rays_d = get_rays(H, W, K, c2w, inverse_y, flip_x, flip_y, mode)
+    viewdirs = rays_d / rays_d.norm(dim=-1, keepdim=True)
+    if ndc:
+        rays_o, rays_d = ndc_rays(H, W, float(K[0][0]), 1., rays_o, rays_d)
+    return rays_o, rays_d, viewdirs
+
+
+def hit_coarse_geo(rays_o, rays_d, xyz_min, xyz_max, near, stepdist, density, mc_min, mc_max,
+                   act_shift, voxel_size_ratio, thres, nearest=False, tie=0.0):
+    """lib/voxurf_fine.py:579-591 with MaskCache.forward (:930-942) on the max-pooled
+    density: hit[r] = some in-box sample of ray r has alpha >= thres.  With tie > 0,
+    returns (hit with thres*(1+tie), hit with thres*(1-tie)): the rays whose answer
+    depends on rounding within that margin differ between the two."""
+    rays_o = rays_o.reshape(-1, 3).float()
+    rays_d = rays_d.reshape(-1, 3).float()
+    pts, mask_out, ray_id = oracle.sample_pts_on_rays(
+        rays_o.numpy(), rays_d.numpy(), np.asarray(xyz_min, np.float32),
+        np.asarray(xyz_max, np.float32), near, 1e9, np.float32(stepdist))[:3]
+    inb = ~mask_out
+    alpha = maskcache_alpha(density.float(), torch.from_numpy(pts[inb]), mc_min, mc_max, act_shift,
+                            voxel_size_ratio, nearest)
+    rid = torch.from_numpy(ray_id[inb])
+
+    def hits(t):
+        h = torch.zeros(rays_o.shape[0], dtype=torch.bool)
+        h[rid[alpha >= t]] = True
+        return h
+    if tie == 0.0:
+        return hits(thres)
+    return hits(thres * (1 + tie)), hits(thres * (1 - tie))
+
+
 # ---- total variation and Adam -----------------------------------------------------------
 
 def tv_add_grad(param, grad, wx, wy, wz, dense_mode, mask=None):

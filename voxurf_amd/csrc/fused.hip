@@ -16,43 +16,9 @@
 // The arithmetic of every sample is the drop-in ops' own (voxel.h), and the
 // transmittance walk is the reference's sequential float recurrence, so the survivor
 // set, weights and alphainv_last equal those of the op-by-op pipeline.
-#include "voxel.h"
+#include "fine.h"
 
 namespace vx {
-
-struct FusedGeom {
-  const float* sdf;       // [X, Y, Z] contiguous
-  GridDesc gs;
-  const float* k0;        // [C, X, Y, Z] with strides
-  GridDesc gk;
-  float mn[3], mx[3];
-  float stepdist, voxel_size, inv_s, thres, near_, far_;
-  // optional MaskCache (coarse occupancy); mc == nullptr disables it
-  const float* mc;
-  GridDesc gm;
-  float mc_shift, mc_ratio, mc_thres, mc_mn[3], mc_mx[3];
-  int mc_nearest;
-};
-
-// MaskCache.forward for one point (lib/voxurf_fine.py:930-942), as vx_maskcache_alpha.
-__device__ __forceinline__ bool maskcache_pass(const FusedGeom& G, const float q[3]) {
-#pragma clang fp contract(off)
-  const float fx = world_to_index(q[0], G.mc_mn[0], G.mc_mx[0] - G.mc_mn[0], G.gm.X);
-  const float fy = world_to_index(q[1], G.mc_mn[1], G.mc_mx[1] - G.mc_mn[1], G.gm.Y);
-  const float fz = world_to_index(q[2], G.mc_mn[2], G.mc_mx[2] - G.mc_mn[2], G.gm.Z);
-  float d;
-  if (G.mc_nearest) {
-    const int x = sat_f2i(rintf(fx)), y = sat_f2i(rintf(fy)), z = sat_f2i(rintf(fz));
-    d = in_grid<float>(x, y, z, G.gm) ? G.mc[x * G.gm.sx + y * G.gm.sy + z * G.gm.sz] : 0.f;
-  } else {
-    Tri<float> t;
-    t.init(fx, fy, fz);
-    d = tri_gather(G.mc, G.gm, t, 0);
-  }
-  const float x = d + G.mc_shift;
-  const float sp = x > 20.f ? x : log1pf(expf(x));
-  return 1.f - expf(-sp * G.mc_ratio) >= G.mc_thres;
-}
 
 __device__ __forceinline__ uint64_t lanemask_lt(int lane) {
   return lane ? ((~0ull) >> (64 - lane)) : 0ull;
@@ -162,14 +128,6 @@ __device__ __forceinline__ int kth_set_bit(uint64_t m, int k) {
     }
   }
   return pos;
-}
-
-__device__ __forceinline__ void ray_setup(const FusedGeom& G, const float* o, const float* d,
-                                          float* start, float* dir, int64_t& n) {
-  float tmin, tmax;
-  ray_t_minmax<float>(o, d, G.mn, G.mx, G.near_, G.far_, tmin, tmax);
-  n = ray_n_samples<float>(d, tmin, tmax, G.stepdist);
-  ray_start_dir<float>(o, d, tmin, start, dir);
 }
 
 // Forward.  Outputs per ray: cnt (survivors), last (alphainv_last), stats
@@ -555,65 +513,6 @@ __global__ void __launch_bounds__(1024) k_sum_f32(const float* __restrict__ x, i
 }  // namespace vx
 
 using namespace vx;
-
-static int fused_geom(FusedGeom& G, const float* sdf, const int64_t* sdf_geom, const float* k0,
-                      const int64_t* k0_geom, const float* box, const float* params) {
-  // box = {mn x,y,z, mx x,y,z}; params = {stepdist, voxel_size, inv_s, thres, near, far}
-  G.sdf = sdf;
-  G.k0 = k0;
-  auto desc = [](const int64_t* s, GridDesc& g) {
-    g.C = (int)s[0];
-    g.X = (int)s[1];
-    g.Y = (int)s[2];
-    g.Z = (int)s[3];
-    g.sc = s[4];
-    g.sx = s[5];
-    g.sy = s[6];
-    g.sz = s[7];
-  };
-  desc(sdf_geom, G.gs);
-  if (k0_geom) desc(k0_geom, G.gk);
-  else G.gk = G.gs;
-  VX_CHECK_ARG(G.gs.C == 1, "fused: the SDF grid must have one channel");
-  for (int i = 0; i < 3; ++i) {
-    G.mn[i] = box[i];
-    G.mx[i] = box[3 + i];
-  }
-  G.stepdist = params[0];
-  G.voxel_size = params[1];
-  G.inv_s = params[2];
-  G.thres = params[3];
-  G.near_ = params[4];
-  G.far_ = params[5];
-  G.mc = nullptr;
-  return VX_OK;
-}
-
-// mc_params = {act_shift, voxel_size_ratio, thres, nearest, xyz_min[3], xyz_max[3]}
-static int fused_maskcache(FusedGeom& G, const float* mc, const int64_t* mc_geom,
-                           const float* mc_params) {
-  if (!mc) return VX_OK;
-  VX_CHECK_ARG(mc_geom && mc_params, "fused: mask cache geometry/parameters missing");
-  G.mc = mc;
-  G.gm.C = (int)mc_geom[0];
-  G.gm.X = (int)mc_geom[1];
-  G.gm.Y = (int)mc_geom[2];
-  G.gm.Z = (int)mc_geom[3];
-  G.gm.sc = mc_geom[4];
-  G.gm.sx = mc_geom[5];
-  G.gm.sy = mc_geom[6];
-  G.gm.sz = mc_geom[7];
-  VX_CHECK_ARG(G.gm.C == 1, "fused: the mask-cache density must have one channel");
-  G.mc_shift = mc_params[0];
-  G.mc_ratio = mc_params[1];
-  G.mc_thres = mc_params[2];
-  G.mc_nearest = mc_params[3] != 0.f;
-  for (int i = 0; i < 3; ++i) {
-    G.mc_mn[i] = mc_params[4 + i];
-    G.mc_mx[i] = mc_params[7 + i];
-  }
-  return VX_OK;
-}
 
 extern "C" {
 

@@ -57,6 +57,13 @@ class _VoxurfBase(nn.Module):
             mask_outbbox[~mask_outbbox] |= ~mask
         return ray_pts, ray_id, step_id, mask_outbbox, N_steps
 
+    def hit_coarse_geo(self, rays_o, rays_d, near, far, stepsize, **render_kwargs):
+        """lib/voxurf_fine.py:579-591: whether each ray has an in-box sample (far forced to
+        1e9) that passes the MaskCache."""
+        from .rays import hit_coarse_geo
+        return hit_coarse_geo(self.mask_cache, self.xyz_min, self.xyz_max, rays_o, rays_d, near,
+                              stepsize * self.voxel_size)
+
     def sdf_and_gradient(self, ray_pts):
         sdf, gradient, _ = SdfGradSample.apply(self.sdf.grid, ray_pts, self.xyz_min, self.xyz_max,
                                                 self.voxel_size)
