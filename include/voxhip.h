@@ -217,6 +217,33 @@ int vx_view_hit_coarse_geo(const float* cam, const float* box, const float* para
 int vx_view_compact_rays(const float* cam, const float* rgb, const int64_t* incl, float* rgb_out,
                          float* o_out, float* d_out, float* v_out, void* stream);
 
+/* ---- dense per-step grid operators ---------------------------------------------- */
+
+/* Number of per-block partial sums vx_smooth_grad_tv writes for an X*Y*Z grid. */
+int64_t vx_smooth_grad_tv_partials(int64_t X, int64_t Y, int64_t Z);
+
+/* lib/voxurf_fine.py:412-421 density_total_variation(smooth_grad_tv=weight) on the
+ * 'interpolate' gradient grid (:440-450) with tv_smooth_conv (:204-240, replicate
+ * padding): loss = weight * mean over masked entries of (S(G) - G)^2, S detached.
+ * taps[3] (host): the 1-D factors of the 3x3x3 smoothing kernel.  mask [X,Y,Z] bytes or
+ * NULL (all voxels); mask_count = number of selected voxels.  Writes dgrad [3,X,Y,Z]
+ * (dloss/dG), partials [vx_smooth_grad_tv_partials], loss [1] (if not NULL) and adds
+ * dloss/dsdf into sdf_grad [X,Y,Z] (if not NULL). */
+int vx_smooth_grad_tv(int dtype, const void* sdf, int X, int Y, int Z, double voxel_size,
+                      const double* taps, const uint8_t* mask, int64_t mask_count, double weight,
+                      void* dgrad, void* partials, void* loss, void* sdf_grad, void* stream);
+
+/* lib/voxurf_fine.py:246-258 _gaussian_3dconv applied to [C,X,Y,Z] (each channel):
+ * conv3d with replicate padding by the separable kernel taps (x) taps (x) taps (k odd,
+ * <= 9; host array), or its adjoint (transpose = 1).  tmp: scratch of the input's size. */
+int vx_smooth3d(int dtype, const void* in, void* out, void* tmp, int C, int X, int Y, int Z,
+                const double* taps, int k, int transpose, void* stream);
+
+/* lib/grid.py:60-65 DenseGrid.scale_volume_grid / lib/voxurf_fine.py:280-283:
+ * F.interpolate(..., mode='trilinear', align_corners=True) of [C,X,Y,Z] to [C,X2,Y2,Z2]. */
+int vx_resample_trilinear(int dtype, const void* in, int C, int X, int Y, int Z, void* out, int X2,
+                          int Y2, int Z2, void* stream);
+
 /* Zero n elements (a fill kernel, safe inside captured graphs). */
 int vx_zero(int dtype, void* p, int64_t n, void* stream);
 

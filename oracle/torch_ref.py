@@ -228,6 +228,59 @@ def hit_coarse_geo(rays_o, rays_d, xyz_min, xyz_max, near, stepdist, density, mc
     return hits(thres * (1 + tie)), hits(thres * (1 - tie))
 
 
+# ---- dense per-step grid operators ------------------------------------------------------
+
+def gaussian_3dconv_weight(ksize, sigma, dtype=torch.float32):
+    """lib/voxurf_fine.py:246-258: the normalised ksize^3 Gaussian (weight [1,1,k,k,k])."""
+    x = np.arange(-(ksize // 2), ksize // 2 + 1, 1)
+    y = np.arange(-(ksize // 2), ksize // 2 + 1, 1)
+    z = np.arange(-(ksize // 2), ksize // 2 + 1, 1)
+    xx, yy, zz = np.meshgrid(x, y, z)
+    kernel = np.exp(-(xx ** 2 + yy ** 2 + zz ** 2) / (2 * sigma ** 2))
+    kernel = torch.from_numpy(kernel).to(dtype)
+    return kernel[None, None, ...] / kernel.sum()
+
+
+def gaussian_smooth(grid, ksize, sigma):
+    """The Conv3d of _gaussian_3dconv (padding ksize//2, replicate) on [1,1,X,Y,Z]."""
+    w = gaussian_3dconv_weight(ksize, sigma, grid.dtype)
+    p = ksize // 2
+    return F.conv3d(F.pad(grid, (p,) * 6, mode="replicate"), w)
+
+
+def tv_smooth_weight(sigma=0.0, dtype=torch.float32):
+    """lib/voxurf_fine.py:207-220, 235-240: tv_smooth_conv's 3x3x3 weight."""
+    kernel = np.asarray([
+        [[1, 2, 1], [2, 4, 2], [1, 2, 1]],
+        [[2, 4, 2], [4, 8, 4], [2, 4, 2]],
+        [[1, 2, 1], [2, 4, 2], [1, 2, 1]],
+    ])
+    distance = np.zeros((3, 3, 3))
+    for i in range(3):
+        for j in range(3):
+            for k in range(3):
+                distance[i, j, k] = ((i - 1) ** 2 + (j - 1) ** 2 + (k - 1) ** 2 - 1)
+    kernel0 = kernel * np.exp(-distance * sigma)
+    return torch.from_numpy(kernel0 / kernel0.sum()).to(dtype)[None, None]
+
+
+def smooth_grad_tv(sdf, voxel_size, mask, weight, sigma=0.0):
+    """lib/voxurf_fine.py:412-421, the smooth_grad_tv term of density_total_variation
+    (scaled by `weight`) on the 'interpolate' gradient grid; differentiable in sdf, with
+    the smoothed gradient detached as in the reference.  mask [1,1,X,Y,Z] bool."""
+    gradient = neus_sdf_gradient(sdf, voxel_size)
+    g = gradient.permute(1, 0, 2, 3, 4)
+    sm = F.conv3d(F.pad(g, (1,) * 6, mode="replicate"), tv_smooth_weight(sigma, sdf.dtype))
+    err = sm.detach() - g
+    err = err[mask.repeat(3, 1, 1, 1, 1)] ** 2
+    return err.mean() * weight
+
+
+def scale_volume_grid(grid, new_world_size):
+    """lib/grid.py:60-65."""
+    return F.interpolate(grid, size=tuple(new_world_size), mode="trilinear", align_corners=True)
+
+
 # ---- total variation and Adam -----------------------------------------------------------
 
 def tv_add_grad(param, grad, wx, wy, wz, dense_mode, mask=None):
@@ -355,9 +408,13 @@ def make_state(cfg, dtype=torch.float32, seed=None):
     return st
 
 
-def train_step(state, opt_state, rays_o, rays_d, viewdirs, target, cfg, step, coarse=False):
+def train_step(state, opt_state, rays_o, rays_d, viewdirs, target, cfg, step, coarse=False,
+               tv=None, nonempty_mask=None):
     """One training step (run.py:600-659): forward, MSE loss, backward, TV (coarse),
-    Adam (utils.Adam, betas (0.9, 0.99)).  Mutates state/opt_state; returns a dict."""
+    Adam (utils.Adam, betas (0.9, 0.99)).  Mutates state/opt_state; returns a dict.
+    tv: the fine stage's grid regularisers (run.py:617-622, 641-658) as a dict with
+    weight_tv_density, sdf_tv, smooth_grad_tv, tv_every, tv_from, tv_end, tv_dense_before;
+    nonempty_mask [1,1,X,Y,Z] (all voxels when None)."""
     params = [("sdf", state["sdf"], cfg.lr_sdf), ("k0", state["k0"], cfg.lr_k0)]
     for i, (W, b) in enumerate(state["mlp"]):
         params += [(f"mlp{i}.w", W, cfg.lr_mlp), (f"mlp{i}.b", b, cfg.lr_mlp)]
@@ -367,9 +424,20 @@ def train_step(state, opt_state, rays_o, rays_d, viewdirs, target, cfg, step, co
     fwd = coarse_forward if coarse else fine_forward
     rgb, extra = fwd(state, rays_o, rays_d, viewdirs, cfg)
     loss = F.mse_loss(rgb, target)
+    tv_on = tv is not None and tv["tv_from"] < step < tv["tv_end"] and step % tv["tv_every"] == 0
+    if tv_on and tv["smooth_grad_tv"] > 0:
+        mask = nonempty_mask
+        if mask is None:
+            mask = torch.ones((1, 1, *state["sdf"].shape[-3:]), dtype=torch.bool)
+        loss = loss + tv["weight_tv_density"] * smooth_grad_tv(
+            state["sdf"], cfg.voxel_size, mask, tv["smooth_grad_tv"])
     loss.backward()
     grads = {}
     with torch.no_grad():
+        if tv_on and tv["sdf_tv"] > 0:
+            w = tv["weight_tv_density"] * tv["sdf_tv"] / len(rays_o)
+            w = w * max(state["sdf"].shape[-3:]) / 128
+            tv_add_grad(state["sdf"], state["sdf"].grad, w, w, w, step < tv["tv_dense_before"])
         if cfg.tv_weight > 0:
             tv_add_grad(state["sdf"], state["sdf"].grad, cfg.tv_weight, cfg.tv_weight,
                         cfg.tv_weight, True)

@@ -165,41 +165,40 @@ __global__ void k_sdf_grad_bwd(T* __restrict__ ggrid, GridDesc g, const T* __res
 
 // ---- neus_sdf_gradient 'interpolate' (lib/voxurf_fine.py:440-450) ------------------
 // out[a] = (s[+1] - s[-1]) / 2 / voxel_size along axis a, 0 on that axis' boundary faces.
+// Launched as a 3-D grid (z chunks of 256 threads, y, x): no index division.
 
 template <typename T>
-__global__ void k_dense_grad_fwd(const T* __restrict__ s, int X, int Y, int Z, T vs,
-                                 T* __restrict__ out) {
+__global__ void __launch_bounds__(256) k_dense_grad_fwd(const T* __restrict__ s, int X, int Y, int Z,
+                                                        T vs, T* __restrict__ out) {
 #pragma clang fp contract(off)
+  const int z = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, x = blockIdx.z;
+  if (z >= Z) return;
   const int64_t N = (int64_t)X * Y * Z;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int z = (int)(i % Z), y = (int)(i / Z % Y), x = (int)(i / Z / Y);
-    const int64_t sx = (int64_t)Y * Z, sy = Z;
-    out[i] = (x > 0 && x < X - 1) ? ((s[i + sx] - s[i - sx]) / (T)2) / vs : (T)0;
-    out[N + i] = (y > 0 && y < Y - 1) ? ((s[i + sy] - s[i - sy]) / (T)2) / vs : (T)0;
-    out[2 * N + i] = (z > 0 && z < Z - 1) ? ((s[i + 1] - s[i - 1]) / (T)2) / vs : (T)0;
-  }
+  const int64_t sx = (int64_t)Y * Z, sy = Z;
+  const int64_t i = x * sx + y * sy + z;
+  out[i] = (x > 0 && x < X - 1) ? ((s[i + sx] - s[i - sx]) / (T)2) / vs : (T)0;
+  out[N + i] = (y > 0 && y < Y - 1) ? ((s[i + sy] - s[i - sy]) / (T)2) / vs : (T)0;
+  out[2 * N + i] = (z > 0 && z < Z - 1) ? ((s[i + 1] - s[i - 1]) / (T)2) / vs : (T)0;
 }
 
 // Adjoint: gs[i] += sum_a (g_a[i - e_a] - g_a[i + e_a]) / 2 / vs over interior entries.
 template <typename T>
-__global__ void k_dense_grad_bwd(const T* __restrict__ go, int X, int Y, int Z, T vs,
-                                 T* __restrict__ gs) {
+__global__ void __launch_bounds__(256) k_dense_grad_bwd(const T* __restrict__ go, int X, int Y, int Z,
+                                                        T vs, T* __restrict__ gs) {
 #pragma clang fp contract(off)
+  const int z = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, x = blockIdx.z;
+  if (z >= Z) return;
   const int64_t N = (int64_t)X * Y * Z;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < N;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int z = (int)(i % Z), y = (int)(i / Z % Y), x = (int)(i / Z / Y);
-    const int64_t sx = (int64_t)Y * Z, sy = Z;
-    T acc = 0;
-    if (x - 1 > 0) acc += (go[i - sx] / (T)2) / vs;
-    if (x + 1 < X - 1) acc -= (go[i + sx] / (T)2) / vs;
-    if (y - 1 > 0) acc += (go[N + i - sy] / (T)2) / vs;
-    if (y + 1 < Y - 1) acc -= (go[N + i + sy] / (T)2) / vs;
-    if (z - 1 > 0) acc += (go[2 * N + i - 1] / (T)2) / vs;
-    if (z + 1 < Z - 1) acc -= (go[2 * N + i + 1] / (T)2) / vs;
-    gs[i] += acc;
-  }
+  const int64_t sx = (int64_t)Y * Z, sy = Z;
+  const int64_t i = x * sx + y * sy + z;
+  T acc = 0;
+  if (x - 1 > 0) acc += (go[i - sx] / (T)2) / vs;
+  if (x + 1 < X - 1) acc -= (go[i + sx] / (T)2) / vs;
+  if (y - 1 > 0) acc += (go[N + i - sy] / (T)2) / vs;
+  if (y + 1 < Y - 1) acc -= (go[N + i + sy] / (T)2) / vs;
+  if (z - 1 > 0) acc += (go[2 * N + i - 1] / (T)2) / vs;
+  if (z + 1 < Z - 1) acc -= (go[2 * N + i + 1] / (T)2) / vs;
+  gs[i] += acc;
 }
 
 // ---- total_variation_add_grad (lib/cuda/total_variation_kernel.cu:13-66) ----------
@@ -409,8 +408,9 @@ int vx_sdf_grad_bwd(int dtype, void* ggrid, const int64_t* geom, const void* xyz
 int vx_dense_sdf_gradient_fwd(int dtype, const void* sdf, int X, int Y, int Z, double voxel_size,
                               void* out, void* stream) {
   VX_CHECK_ARG(X >= 1 && Y >= 1 && Z >= 1, "dense gradient: bad shape");
-  const int64_t N = (int64_t)X * Y * Z;
-  GRID_DISPATCH(dtype, k_dense_grad_fwd<scalar_t><<<grid_for(N, 256), 256, 0, ST(stream)>>>(
+  VX_CHECK_ARG(X <= 65535 && Y <= 65535, "dense gradient: X and Y must be <= 65535");
+  const dim3 grid((unsigned)cdiv(Z, 256), (unsigned)Y, (unsigned)X);
+  GRID_DISPATCH(dtype, k_dense_grad_fwd<scalar_t><<<grid, 256, 0, ST(stream)>>>(
                            CP(sdf), X, Y, Z, (scalar_t)voxel_size, MP(out)));
   return launch_status("dense_sdf_gradient_fwd");
 }
@@ -418,8 +418,9 @@ int vx_dense_sdf_gradient_fwd(int dtype, const void* sdf, int X, int Y, int Z, d
 int vx_dense_sdf_gradient_bwd(int dtype, const void* gout, int X, int Y, int Z,
                               double voxel_size, void* gsdf, void* stream) {
   VX_CHECK_ARG(X >= 1 && Y >= 1 && Z >= 1, "dense gradient: bad shape");
-  const int64_t N = (int64_t)X * Y * Z;
-  GRID_DISPATCH(dtype, k_dense_grad_bwd<scalar_t><<<grid_for(N, 256), 256, 0, ST(stream)>>>(
+  VX_CHECK_ARG(X <= 65535 && Y <= 65535, "dense gradient: X and Y must be <= 65535");
+  const dim3 grid((unsigned)cdiv(Z, 256), (unsigned)Y, (unsigned)X);
+  GRID_DISPATCH(dtype, k_dense_grad_bwd<scalar_t><<<grid, 256, 0, ST(stream)>>>(
                            CP(gout), X, Y, Z, (scalar_t)voxel_size, MP(gsdf)));
   return launch_status("dense_sdf_gradient_bwd");
 }

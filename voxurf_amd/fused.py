@@ -14,6 +14,8 @@ One step (run.py:600-659 for the fine model, lib/voxurf_fine.py:620-802):
   6. rgbnet backward
   7. vx_fused_fine_feat_bwd   k0 scatter, normal adjoint
   8. vx_fused_fine_bwd        per-ray backward of the SDF side over the stored walk range
+     (on regulariser steps, the smooth-gradient and SDF TV terms are added to the SDF
+     gradient here: voxurf_amd.smooth, vx_total_variation_add_grad)
   9. Adam (utils.Adam semantics) over sdf, k0 and rgbnet, clearing the grid gradients.
 
 Streams.  The k0 Adam pass (the largest HBM sweep of the step) runs on a second stream
@@ -31,8 +33,9 @@ import torch
 
 from . import _lib
 from ._lib import call, ptr, stream_of
-from . import mlp_hip
+from . import mlp_hip, smooth
 from .mlp import RgbNet
+from .scene import tv_active
 from .total_variation_cuda import grid_geom
 
 # rgbnet input row width: [k0 (12) | normal (3) | view PE (27)] = 42 features, zero
@@ -121,6 +124,13 @@ class FusedFineTrainer:
         self.last_live_pages = None
 
     mask_cache = None
+    # Grid regularisers of the fine stage (run.py:617-622, 641-658): None, or a dict as
+    # voxurf_amd.scene.FINE_TV; applied on the steps scene.tv_active selects.
+    tv = None
+    # [X, Y, Z] (or [1, 1, X, Y, Z]) bool: the voxels the smooth-gradient term averages
+    # over (Voxurf.nonempty_mask, lib/voxurf_fine.py:354-367); None counts every voxel.
+    nonempty_mask = None
+    _sgtv = None
 
     @property
     def k0(self):
@@ -280,6 +290,7 @@ class FusedFineTrainer:
              self.params.data_ptr(), ptr(viewdirs), n, self.cap, ptr(self.s4), ptr(self.g4),
              ptr(self.stats), ptr(self.cnt), ptr(self.off), ptr(self.last), ptr(gw),
              ptr(self.glast), ptr(self.S), ptr(dgf), ptr(self.sdf_grad), st)
+        tv_loss = self._regularise(n, k0_step) if tv_active(self.tv, k0_step) else None
         if profile:
             self.grads_snapshot = dict(sdf=self.sdf_grad.clone(), k0=k0_grad_snapshot,
                                        mlp=[g.clone() for g in self.mlp_grads])
@@ -288,12 +299,40 @@ class FusedFineTrainer:
         # mean squared error over the batch's N x 3 colour components (run.py:604)
         loss = torch.empty(1, dtype=torch.float32, device=self.dev)
         call("vx_sum_f32", ptr(self.loss_r), n, 1.0 / (3.0 * n), ptr(loss), st)
+        if tv_loss is not None:
+            loss += tv_loss
         if profile:
             torch.cuda.synchronize()
             s = self.stats[:n].sum(0).tolist()
             self.last_counts = {"m0": int(s[2]), "m0_processed": int(s[0]), "m1b": int(s[1]),
                                 "m2": m}
         return dict(loss=loss, rgb_marched=self.rgb[:n], n_samples=m)
+
+    def _regularise(self, n, step):
+        """Add the fine stage's grid regularisers to sdf_grad (run.py:617-622, 641-658):
+        the smooth-gradient term (its value is returned, a 1-element tensor, or None)
+        and the SDF total variation."""
+        tv, cfg = self.tv, self.cfg
+        out = None
+        if tv["smooth_grad_tv"] > 0:
+            mask = self.nonempty_mask
+            if self._sgtv is None or self._sgtv[0] is not mask:
+                shape = tuple(self.sdf.shape[-3:])
+                m = None if mask is None else \
+                    mask.reshape(shape).to(self.dev, torch.bool).contiguous()
+                count = shape[0] * shape[1] * shape[2] if m is None else int(m.sum())
+                self._sgtv = (mask, m, count, smooth.smooth_grad_tv_workspace(self.sdf))
+            _, m, count, ws = self._sgtv
+            out = torch.empty(1, dtype=torch.float32, device=self.dev)
+            smooth.smooth_grad_tv_add_grad(self.sdf, self.sdf_grad, cfg.voxel_size, m,
+                                           tv["weight_tv_density"] * tv["smooth_grad_tv"],
+                                           mask_count=count, loss=out, workspace=ws)
+        if tv["sdf_tv"] > 0:
+            w = tv["weight_tv_density"] * tv["sdf_tv"] / n * max(self.sdf.shape[-3:]) / 128
+            call("vx_total_variation_add_grad", 0, ptr(self.sdf), ptr(self.sdf_grad),
+                 self.sdf_geom.data_ptr(), w, w, w, int(step < tv["tv_dense_before"]),
+                 stream_of(self.sdf))
+        return out
 
     def state_dict(self):
         sd = {"sdf.grid": self.sdf, "k0.grid": self.k0}

@@ -10,7 +10,6 @@ scattered words.  The tensor's shape, indexing and state-dict entry are unchange
 """
 import torch
 import torch.nn as nn
-import torch.nn.functional as F
 
 from . import total_variation_cuda
 from .functional import grid_sample
@@ -58,12 +57,16 @@ class DenseGrid(nn.Module):
         return out
 
     def scale_volume_grid(self, new_world_size):
+        """Resample the grid to new_world_size (lib/grid.py:60-65: trilinear,
+        align_corners=True) on the HIP kernel vx_resample_trilinear."""
+        from .smooth import resample_trilinear
         if self.channels == 0:
-            self.grid = nn.Parameter(torch.zeros([1, self.channels, *new_world_size]))
+            self.grid = nn.Parameter(torch.zeros([1, self.channels, *new_world_size],
+                                                 device=self.grid.device))
         else:
-            self.grid = nn.Parameter(_layout(
-                F.interpolate(self.grid.data, size=tuple(new_world_size), mode="trilinear",
-                              align_corners=True), self.channels_last))
+            self.grid = nn.Parameter(_layout(resample_trilinear(self.grid.data, new_world_size),
+                                             self.channels_last))
+        self.world_size = torch.as_tensor(new_world_size)
 
     def total_variation_add_grad(self, wx, wy, wz, dense_mode, mask=None):
         """Add the TV gradient to grid.grad in place (lib/grid.py:67-78)."""

@@ -55,6 +55,19 @@ COARSE = SceneConfig(R=96, feat_dim=3, n_rays=8192, inv_s=64.0, lr_sdf=0.1, lr_k
 # Plain fine step: 256^3 SDF + 12-channel features, rgbnet 42->192x3->3, s = 256.
 FINE = SceneConfig(R=256, feat_dim=12, n_rays=8192, inv_s=256.0)
 
+# Grid regularisers of the fine stage (configs/dtu_e2e/fine.py:34-45, run.py:617-622 and
+# 641-658): on steps with tv_from < step < tv_end and step % tv_every == 0, the loss gains
+# weight_tv_density * smooth_grad_tv * mean((S(G) - G)^2) over the non-empty voxels, and
+# the SDF gradient gains the TV term of weight weight_tv_density * sdf_tv / N_rays * R/128
+# (dense while step < tv_dense_before).
+FINE_TV = dict(weight_tv_density=0.01, sdf_tv=0.1, smooth_grad_tv=0.05, tv_every=3, tv_from=0,
+               tv_end=30000, tv_dense_before=20000)
+
+
+def tv_active(tv, step):
+    """run.py:617, 641: whether the regularisers apply at (1-based) step `step`."""
+    return tv is not None and tv["tv_from"] < step < tv["tv_end"] and step % tv["tv_every"] == 0
+
 
 def _gen(seed, tag):
     g = torch.Generator()
