@@ -56,3 +56,44 @@ def test_model_state_dict_round_trip():
     m2.load_state_dict(sd)
     for k, v in m2.state_dict().items():
         assert torch.equal(v, sd[k]), k
+
+
+def test_checkpoint_kwargs_round_trip_and_safe_loading(tmp_path):
+    """model_kwargs <-> SceneConfig; a reference-style checkpoint (numpy arrays in
+    model_kwargs) loads with weights_only; a file holding any other object is refused."""
+    import pickle
+
+    import numpy as np
+    import pytest
+
+    from voxurf_amd import checkpoint as ck
+    cfg = scene.FINE
+    kw = ck.model_kwargs(cfg)
+    back = ck.config_from_kwargs(kw)
+    for f in ("R", "feat_dim", "inv_s", "stepsize", "fast_color_thres", "mlp_width",
+              "mlp_depth", "view_pe"):
+        assert getattr(back, f) == getattr(cfg, f), f
+    ref_style = {"global_step": 7,
+                 "model_kwargs": dict(kw, xyz_min=np.array([-1., -1., -1.], np.float32),
+                                      xyz_max=np.array([1., 1., 1.], np.float32)),
+                 "MaskCache_kwargs": None,
+                 "model_state_dict": ck.model_state_dict(torch.zeros(1, 1, 4, 4, 4),
+                                                         torch.zeros(1, 12, 4, 4, 4),
+                                                         RgbNet(42, 192, 4).state_dict())}
+    path = tmp_path / "ref.tar"
+    torch.save(ref_style, path)
+    loaded = ck._safe_load(path)
+    assert loaded["global_step"] == 7
+    assert ck.config_from_kwargs(loaded["model_kwargs"]).R == 256
+    keys = set(loaded["model_state_dict"])
+    assert {"xyz_min", "xyz_max", "sdf.grid", "k0.grid", "rgbnet.0.weight", "rgbnet.2.0.weight",
+            "rgbnet.4.bias"} <= keys
+
+    class Payload:
+        def __reduce__(self):
+            return (print, ("executed",))
+    bad = tmp_path / "bad.tar"
+    with open(bad, "wb") as f:
+        pickle.dump({"model_kwargs": Payload()}, f)
+    with pytest.raises(Exception):
+        ck._safe_load(bad)

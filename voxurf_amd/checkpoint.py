@@ -1,0 +1,225 @@
+"""Checkpoints of the fine stage in the reference's format, and loading fitted scenes
+into the trainer or the server.
+
+The file is the dictionary run.py:786-794 saves: global_step, model_kwargs,
+MaskCache_kwargs, model_state_dict (the nn.Module key names: xyz_min, xyz_max,
+sdf.grid, sdf.xyz_min, sdf.xyz_max, k0.*, rgbnet.<i>.weight/bias, and mask_cache.* when a
+MaskCache is attached) and optimizer_state_dict (torch.optim layout: per-parameter
+step / exp_avg / exp_avg_sq, and one param group per learning rate, as utils.Adam keeps
+them).  model_kwargs holds plain Python values; besides the reference's keys it records
+the sampling and shading settings the fused kernels need (stepsize, s_val, viewbase_pe).
+
+Loading uses torch.load(weights_only=True) only.  The reference stores xyz_min and
+xyz_max in model_kwargs as numpy arrays; numpy's array reconstruction is allowed for
+that (and nothing else is).  load_checkpoint follows lib/utils.py:234-251, including
+zero-padding a narrower rgbnet.0.weight into the model's first layer.
+"""
+import numpy as np
+import torch
+
+from .maskcache import MaskCache
+from .scene import SceneConfig
+
+
+def _safe_load(path_or_file):
+    """torch.load(weights_only=True), with numpy array reconstruction allowed (the
+    reference's model_kwargs hold numpy arrays)."""
+    allowed = [np.ndarray, np.dtype]
+    core = getattr(np, "_core", None) or np.core
+    allowed.append(core.multiarray._reconstruct)
+    for name in ("float32", "float64", "int64", "int32", "bool_"):
+        allowed.append(type(np.dtype(name)))
+    with torch.serialization.safe_globals(allowed):
+        return torch.load(path_or_file, weights_only=True, map_location="cpu")
+
+
+def _as_list(v):
+    return [float(x) for x in (v.tolist() if hasattr(v, "tolist") else v)]
+
+
+def model_kwargs(cfg: SceneConfig):
+    """model_kwargs of a fine-stage scene (keys of Voxurf.get_kwargs where they apply)."""
+    return {
+        "xyz_min": [-1.0, -1.0, -1.0],
+        "xyz_max": [1.0, 1.0, 1.0],
+        "num_voxels": cfg.R ** 3,
+        "num_voxels_base": cfg.R ** 3,
+        "fast_color_thres": cfg.fast_color_thres,
+        "rgbnet_dim": cfg.feat_dim,
+        "rgbnet_width": cfg.mlp_width,
+        "rgbnet_depth": cfg.mlp_depth,
+        "viewbase_pe": cfg.view_pe,
+        "stepsize": cfg.stepsize,
+        "s_val": 1.0 / cfg.inv_s,
+    }
+
+
+def config_from_kwargs(kw, n_rays=8192):
+    """SceneConfig of a checkpoint's model_kwargs (the grid side from num_voxels over the
+    [-1, 1]^3 box, as _set_grid_resolution computes it)."""
+    mn, mx = _as_list(kw["xyz_min"]), _as_list(kw["xyz_max"])
+    if mn != [-1.0, -1.0, -1.0] or mx != [1.0, 1.0, 1.0]:
+        raise RuntimeError("the fused kernels expect the [-1, 1]^3 scene box")
+    vs = (8.0 / kw["num_voxels"]) ** (1.0 / 3.0)
+    R = int(round(2.0 / vs))
+    return SceneConfig(R=R, feat_dim=int(kw["rgbnet_dim"]), n_rays=n_rays,
+                       inv_s=1.0 / float(kw.get("s_val", 1.0 / 256.0)),
+                       stepsize=float(kw.get("stepsize", 0.5)),
+                       fast_color_thres=float(kw.get("fast_color_thres", 1e-4)),
+                       mlp_width=int(kw.get("rgbnet_width", 192)),
+                       mlp_depth=int(kw.get("rgbnet_depth", 4)),
+                       view_pe=int(kw.get("viewbase_pe", 4)))
+
+
+def _box():
+    return torch.tensor([-1.0, -1.0, -1.0]), torch.tensor([1.0, 1.0, 1.0])
+
+
+def model_state_dict(sdf, k0, rgbnet_state, mask_cache=None):
+    """The fine model's state dict (CPU tensors, reference layout)."""
+    mn, mx = _box()
+    sd = {"xyz_min": mn, "xyz_max": mx,
+          "sdf.xyz_min": mn.clone(), "sdf.xyz_max": mx.clone(),
+          "sdf.grid": sdf.detach().cpu().contiguous(),
+          "k0.xyz_min": mn.clone(), "k0.xyz_max": mx.clone(),
+          "k0.grid": k0.detach().cpu().contiguous()}
+    for k, v in rgbnet_state.items():
+        sd["rgbnet." + k] = v.detach().cpu()
+    if mask_cache is not None:
+        for k, v in mask_cache.state_dict().items():
+            sd["mask_cache." + k] = v.detach().cpu()
+    return sd
+
+
+def maskcache_kwargs(mc):
+    if mc is None:
+        return None
+    return {"xyz_min": mc.xyz_min.cpu().tolist(), "xyz_max": mc.xyz_max.cpu().tolist(),
+            "act_shift": mc.act_shift, "voxel_size_ratio": mc.voxel_size_ratio,
+            "nearest": mc.nearest, "mask_cache_thres": mc.mask_cache_thres}
+
+
+def _optimizer_state(trainer):
+    """torch.optim-style state of the trainer's Adam moments: groups sdf, k0, rgbnet."""
+    state, groups = {}, []
+    m, v, step, lr = trainer.adam["sdf"]
+    state[0] = {"step": step, "exp_avg": m.detach().cpu(), "exp_avg_sq": v.detach().cpu()}
+    groups.append({"params": [0], "lr": lr, "betas": (0.9, 0.99), "eps": 1e-8})
+    m, v, step, lr = trainer.adam["k0"]
+    state[1] = {"step": step, "exp_avg": m.detach().cpu().contiguous(),
+                "exp_avg_sq": v.detach().cpu().contiguous()}
+    groups.append({"params": [1], "lr": lr, "betas": (0.9, 0.99), "eps": 1e-8})
+    m, v, step, lr = trainer.adam["mlp"]
+    idx, off = [], 0
+    for i, p in enumerate(trainer.rgbnet.parameters()):
+        n = p.numel()
+        state[2 + i] = {"step": step, "exp_avg": m[off:off + n].view_as(p).detach().cpu(),
+                        "exp_avg_sq": v[off:off + n].view_as(p).detach().cpu()}
+        idx.append(2 + i)
+        off += n
+    groups.append({"params": idx, "lr": lr, "betas": (0.9, 0.99), "eps": 1e-8})
+    return {"state": state, "param_groups": groups}
+
+
+def save_checkpoint(path, trainer, global_step, with_optimizer=True):
+    """Write the trainer's scene as run.py:786-794 does (k0 read after its pending
+    update)."""
+    ckpt = {
+        "global_step": int(global_step),
+        "model_kwargs": model_kwargs(trainer.cfg),
+        "MaskCache_kwargs": maskcache_kwargs(trainer.mask_cache),
+        "model_state_dict": model_state_dict(trainer.sdf, trainer.k0, trainer.rgbnet.state_dict(),
+                                             trainer.mask_cache),
+    }
+    if with_optimizer:
+        ckpt["optimizer_state_dict"] = _optimizer_state(trainer)
+    torch.save(ckpt, path)
+    return ckpt
+
+
+def _rgbnet_keys(depth):
+    """Module paths of the rgbnet Linears (lib/voxurf_fine.py:141-149): Linear, ReLU,
+    (depth - 2) x Sequential(Linear, ReLU), Linear."""
+    return ["0"] + [f"{2 + i}.0" for i in range(depth - 2)] + [f"{depth}"]
+
+
+def _rgbnet_params(sd, depth):
+    """[(W, b), ...] of a state dict's rgbnet.* entries."""
+    return [(sd[f"rgbnet.{k}.weight"], sd[f"rgbnet.{k}.bias"]) for k in _rgbnet_keys(depth)]
+
+
+def _mask_cache_from(sd, kw, device):
+    if "mask_cache.density" not in sd or kw is None:
+        return None
+    return MaskCache.from_pooled(sd["mask_cache.density"], sd["mask_cache.xyz_min"],
+                                 sd["mask_cache.xyz_max"], kw["act_shift"],
+                                 kw["voxel_size_ratio"], kw.get("mask_cache_thres", 1e-3),
+                                 kw.get("nearest", False)).to(device)
+
+
+def load_checkpoint(trainer, ckpt_path, no_reload_optimizer=False, strict=True):
+    """lib/utils.py:234-251 for the fused trainer: loads grids, rgbnet (a narrower
+    rgbnet.0.weight is zero-padded into the first layer) and, unless
+    no_reload_optimizer, the Adam state; returns (trainer, global_step)."""
+    ckpt = _safe_load(ckpt_path)
+    sd = dict(ckpt["model_state_dict"])
+    W0 = trainer.rgbnet.linears()[0].weight
+    if W0.shape != sd["rgbnet.0.weight"].shape:
+        tmp = torch.zeros(W0.shape)
+        h = sd["rgbnet.0.weight"].shape[-1]
+        tmp[:, :h] = sd["rgbnet.0.weight"]
+        sd["rgbnet.0.weight"] = tmp
+    for key in ("sdf.grid", "k0.grid"):
+        if key not in sd:
+            raise RuntimeError(f"checkpoint has no {key}")
+    trainer.load_grids(sd["sdf.grid"], sd["k0.grid"])
+    # copies into the trainer's parameter storage (views of its flat buffer)
+    trainer.rgbnet.load_state_dict({k[len("rgbnet."):]: v for k, v in sd.items()
+                                    if k.startswith("rgbnet.")}, strict=True)
+    mc = _mask_cache_from(sd, ckpt.get("MaskCache_kwargs"), trainer.dev)
+    if mc is not None:
+        trainer.mask_cache = mc
+    if not no_reload_optimizer and "optimizer_state_dict" in ckpt:
+        try:
+            trainer.load_optimizer_state(ckpt["optimizer_state_dict"])
+        except (KeyError, RuntimeError, ValueError) as e:
+            print("Failed to load optimizer state dict")
+            if strict:
+                raise ValueError(str(e)) from e
+            print("Skip!")
+    return trainer, int(ckpt["global_step"])
+
+
+def load_trainer(ckpt_path, device="cuda:0", mlp_dtype=torch.float32, n_rays=8192):
+    """A FusedFineTrainer built from a checkpoint's model_kwargs and loaded from it
+    (lib/utils.py:278-300 load_model, for the fine stage)."""
+    from .fused import FusedFineTrainer
+    ckpt = _safe_load(ckpt_path)
+    cfg = config_from_kwargs(ckpt["model_kwargs"], n_rays)
+    sd = ckpt["model_state_dict"]
+    t = FusedFineTrainer(cfg, sd["sdf.grid"], sd["k0.grid"],
+                         _rgbnet_params(sd, cfg.mlp_depth), device=device, mlp_dtype=mlp_dtype)
+    mc = _mask_cache_from(sd, ckpt.get("MaskCache_kwargs"), t.dev)
+    if mc is not None:
+        t.mask_cache = mc
+    if "optimizer_state_dict" in ckpt:
+        t.load_optimizer_state(ckpt["optimizer_state_dict"])
+    return t, int(ckpt["global_step"])
+
+
+def load_scene(ckpt_path, device="cuda:0", mlp_dtype=torch.float32):
+    """A serve.ResidentScene of a checkpoint (grids, rgbnet, MaskCache, s_val)."""
+    from .serve import ResidentScene
+    ckpt = _safe_load(ckpt_path)
+    cfg = config_from_kwargs(ckpt["model_kwargs"])
+    sd = ckpt["model_state_dict"]
+    scene = ResidentScene(cfg, sd["sdf.grid"], sd["k0.grid"], _rgbnet_params(sd, cfg.mlp_depth),
+                          device=device, mlp_dtype=mlp_dtype)
+    mc = _mask_cache_from(sd, ckpt.get("MaskCache_kwargs"), scene.dev)
+    if mc is not None:
+        scene.mask_cache = mc
+    return scene
+
+
+__all__ = ["save_checkpoint", "load_checkpoint", "load_trainer", "load_scene", "model_kwargs",
+           "config_from_kwargs", "model_state_dict"]

@@ -334,6 +334,41 @@ class FusedFineTrainer:
                  stream_of(self.sdf))
         return out
 
+    @torch.no_grad()
+    def load_grids(self, sdf, k0):
+        """Replace the SDF and feature grids (same shapes) in place."""
+        k0_dst = self.k0  # waits for a pending k0 update
+        if sdf.shape != self.sdf.shape or k0.shape != k0_dst.shape:
+            raise RuntimeError("load_grids: grid shapes differ from the trainer's")
+        self.sdf.copy_(sdf)
+        k0_dst.copy_(k0)
+
+    @torch.no_grad()
+    def load_optimizer_state(self, osd):
+        """Adam moments and step counts from a torch.optim-style state dict with the
+        groups sdf, k0, rgbnet (the layout checkpoint.save_checkpoint writes)."""
+        groups, state = osd["param_groups"], osd["state"]
+        if len(groups) != 3:
+            raise ValueError("optimizer state: expected the groups sdf, k0 and rgbnet")
+        self.k0  # waits for a pending k0 update
+        for name, g in zip(("sdf", "k0"), groups[:2]):
+            st = state[g["params"][0]]
+            self.adam[name][0].copy_(st["exp_avg"])
+            self.adam[name][1].copy_(st["exp_avg_sq"])
+            self.adam[name][2] = int(st["step"])
+        m, v = self.adam["mlp"][0], self.adam["mlp"][1]
+        off, step = 0, 0
+        for i in groups[2]["params"]:
+            st = state[i]
+            n = st["exp_avg"].numel()
+            m[off:off + n].copy_(st["exp_avg"].reshape(-1))
+            v[off:off + n].copy_(st["exp_avg_sq"].reshape(-1))
+            step = int(st["step"])
+            off += n
+        if off != m.numel():
+            raise ValueError("optimizer state: rgbnet moments do not match the network")
+        self.adam["mlp"][2] = step
+
     def state_dict(self):
         sd = {"sdf.grid": self.sdf, "k0.grid": self.k0}
         for k, v in self.rgbnet.state_dict().items():

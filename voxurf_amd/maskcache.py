@@ -36,6 +36,27 @@ class MaskCache(nn.Module):
         self.voxel_size_ratio = float(voxel_size_ratio)
         self.nearest = bool(nearest)
 
+    @classmethod
+    def from_pooled(cls, density, xyz_min, xyz_max, act_shift, voxel_size_ratio,
+                    mask_cache_thres=1e-3, nearest=False):
+        """A MaskCache whose density is already max-pooled (its own `density` buffer, as
+        a model's state dict holds it under mask_cache.density)."""
+        mc = cls(density=torch.zeros(1, 1, 1, 1, 1), xyz_min=xyz_min, xyz_max=xyz_max,
+                 act_shift=act_shift, voxel_size_ratio=voxel_size_ratio,
+                 mask_cache_thres=mask_cache_thres, nearest=nearest)
+        mc.density = density.float().contiguous().clone()
+        return mc
+
+    @torch.no_grad()
+    def lattice_mask(self, xyz_min, xyz_max, world_size):
+        """Voxurf._set_nonempty_mask (lib/voxurf_fine.py:354-367): the mask at the lattice
+        points of a grid of world_size over [xyz_min, xyz_max], [X, Y, Z] bool."""
+        dev = self.density.device
+        axes = [torch.linspace(float(a), float(b), int(n), device=dev)
+                for a, b, n in zip(xyz_min, xyz_max, world_size)]
+        pts = torch.stack(torch.meshgrid(*axes, indexing="ij"), -1)
+        return self(pts)
+
     @torch.no_grad()
     def forward(self, xyz):
         shape = xyz.shape[:-1]

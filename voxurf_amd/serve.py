@@ -97,6 +97,8 @@ class ResidentScene:
         out = torch.empty((n, 3), dtype=torch.float32, device=self.dev)
         if n == 0:
             return out
+        if ws.cap < self.cap:
+            raise RuntimeError(f"workspace holds {ws.cap} samples per ray, the scene needs {self.cap}")
         ws.ensure(n)
         st = stream_of(rays_o)
         call("vx_fused_fine_fwd", ptr(self.sdf), self.sdf_geom.data_ptr(), self.box.data_ptr(),
