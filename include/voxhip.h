@@ -244,6 +244,34 @@ int vx_smooth3d(int dtype, const void* in, void* out, void* tmp, int C, int X, i
 int vx_resample_trilinear(int dtype, const void* in, int C, int X, int Y, int Z, void* out, int X2,
                           int Y2, int Z2, void* stream);
 
+/* ---- mesh extraction (lib/voxurf_fine.py:894-910, lib/dvgo_ori.py:679-703) --------- */
+
+/* lib/dvgo_ori.py:679-693 extract_fields with Voxurf.grid_sampler as query: u[i,j,k] =
+ * scale * trilinear(sdf) at (linspace(bmin, bmax, n)[i], ...) (float32 linspace).  box =
+ * {grid xyz_min, xyz_max}, bounds = {bmin, bmax} (host); u [n0, n1, n2] float32. */
+int vx_extract_fields(const float* sdf, const int64_t* geom, const float* box, const float* bounds,
+                      int n0, int n1, int n2, float scale, float* u, void* stream);
+
+/* Marching-cubes tables (host memory): count [256] triangles per corner configuration
+ * (bit c set when corner c = (c&1, c>>1&1, c>>2&1) is below the threshold), edges
+ * [256 * 3 * vx_mc_max_tris()] lattice-edge ids of each triangle.  No counterpart in the
+ * reference (which calls PyMCubes, lib/dvgo_ori.py:698); generated, not stored. */
+int vx_mc_tables(int8_t* count, int8_t* edges);
+int vx_mc_max_tris(void);
+
+/* lib/dvgo_ori.py:698 mcubes.marching_cubes(u, iso), pass 1: per lattice point, vcnt =
+ * crossing lattice edges it owns (+x, +y, +z), tcnt = triangles of the cube at it
+ * (count/edges: device copies of vx_mc_tables). */
+int vx_mc_count(const float* u, int nx, int ny, int nz, float iso, const int8_t* count,
+                const int8_t* edges, int64_t* vcnt, int64_t* tcnt, void* stream);
+
+/* Pass 2: vincl/tincl = inclusive scans of vcnt/tcnt; verts [V, 3] float64 (index space,
+ * or world space v / (n - 1) * (bmax - bmin) + bmin when bounds = {bmin, bmax} host
+ * doubles is given, as lib/dvgo_ori.py:699-702), faces [F, 3] int64. */
+int vx_mc_emit(const float* u, int nx, int ny, int nz, float iso, const int8_t* count,
+               const int8_t* edges, const int64_t* vincl, const int64_t* tincl,
+               const double* bounds, double* verts, int64_t* faces, void* stream);
+
 /* Zero n elements (a fill kernel, safe inside captured graphs). */
 int vx_zero(int dtype, void* p, int64_t n, void* stream);
 

@@ -281,6 +281,80 @@ def scale_volume_grid(grid, new_world_size):
     return F.interpolate(grid, size=tuple(new_world_size), mode="trilinear", align_corners=True)
 
 
+# ---- mesh extraction --------------------------------------------------------------------
+
+def extract_fields(sdf_grid, xyz_min, xyz_max, bound_min, bound_max, resolution):
+    """lib/dvgo_ori.py:679-693 with query_func = Voxurf.grid_sampler(pts, -sdf_grid)
+    (lib/voxurf_fine.py:903), as one lattice (the chunking does not change values)."""
+    X = torch.linspace(float(bound_min[0]), float(bound_max[0]), resolution)
+    Y = torch.linspace(float(bound_min[1]), float(bound_max[1]), resolution)
+    Z = torch.linspace(float(bound_min[2]), float(bound_max[2]), resolution)
+    xx, yy, zz = torch.meshgrid(X, Y, Z, indexing="ij")
+    pts = torch.cat([xx.reshape(-1, 1), yy.reshape(-1, 1), zz.reshape(-1, 1)], dim=-1)
+    val = dense_grid_forward(-sdf_grid, pts, torch.as_tensor(xyz_min, dtype=torch.float32),
+                             torch.as_tensor(xyz_max, dtype=torch.float32))
+    return val.reshape(resolution, resolution, resolution)
+
+
+def mc_vertices(u, iso):
+    """Brute force of marching cubes' vertex set: one vertex on every lattice edge whose
+    end values lie on different sides of iso (one below, value < iso), at
+    p1 + (iso - f1) / (f2 - f1) along the edge, float64.  [V, 3] in lexicographic order."""
+    u = np.asarray(u, dtype=np.float32)
+    out = []
+    b = u < iso
+    for ax in range(3):
+        sl0 = [slice(None)] * 3
+        sl1 = [slice(None)] * 3
+        sl0[ax] = slice(0, u.shape[ax] - 1)
+        sl1[ax] = slice(1, u.shape[ax])
+        cross = b[tuple(sl0)] != b[tuple(sl1)]
+        idx = np.argwhere(cross)
+        f1 = u[tuple(sl0)][cross].astype(np.float64)
+        f2 = u[tuple(sl1)][cross].astype(np.float64)
+        v = idx.astype(np.float64)
+        v[:, ax] += (np.float64(np.float32(iso)) - f1) / (f2 - f1)
+        out.append(v)
+    v = np.concatenate(out, 0)
+    return v[np.lexsort((v[:, 2], v[:, 1], v[:, 0]))]
+
+
+def mc_assemble(u, iso, count, edges):
+    """Marching cubes on the CPU with a given triangle table (count [256], edges
+    [256, 3k]; corner c = (c&1, c>>1&1, c>>2&1), edge 4*axis + (b0 | b1<<1)): the
+    triangles as triples of vertex positions (float64 [F, 3, 3]).  The reference's
+    PyMCubes is not installed, so this checks the GPU assembly against the same table;
+    the table itself is pinned by mesh invariants."""
+    u = np.asarray(u, dtype=np.float32)
+    nx, ny, nz = u.shape
+    b = u < iso
+    tris = []
+    for i in range(nx - 1):
+        for j in range(ny - 1):
+            for k in range(nz - 1):
+                ci = 0
+                for c in range(8):
+                    if b[i + (c & 1), j + (c >> 1 & 1), k + (c >> 2 & 1)]:
+                        ci |= 1 << c
+                for t in range(int(count[ci])):
+                    tri = []
+                    for r in range(3):
+                        e = int(edges[ci][3 * t + r])
+                        ax = e >> 2
+                        others = [a for a in range(3) if a != ax]
+                        p = [i, j, k]
+                        p[others[0]] += e & 1
+                        p[others[1]] += e >> 1 & 1
+                        q = list(p)
+                        q[ax] += 1
+                        f1, f2 = np.float64(u[tuple(p)]), np.float64(u[tuple(q)])
+                        v = np.array(p, dtype=np.float64)
+                        v[ax] += (np.float64(np.float32(iso)) - f1) / (f2 - f1)
+                        tri.append(v)
+                    tris.append(tri)
+    return np.asarray(tris, dtype=np.float64).reshape(-1, 3, 3)
+
+
 # ---- total variation and Adam -----------------------------------------------------------
 
 def tv_add_grad(param, grad, wx, wy, wz, dense_mode, mask=None):
