@@ -37,12 +37,13 @@ def parse():
                     help="run the k0 Adam pass in order on the main stream (per-kernel timing)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--mode", choices=["train", "coarse", "recompute", "precision", "serve",
-                                       "kernels"],
+                                       "kernels", "next_rows"],
                     default="train",
                     help="train: the plain fine step (the headline line); coarse: the plain "
                          "coarse step; recompute: 65536-ray fine step, per-ray recomputation "
                          "vs store-all; precision: BF16 rgbnet vs FP32 vs FP64; serve: the "
-                         "render/SDF-query server vs sequential per-request rendering")
+                         "render/SDF-query server vs sequential per-request rendering; "
+                         "next_rows: ray pre-filter, regularisers, mesh extraction")
     ap.add_argument("--requests", type=int, default=300)
     ap.add_argument("--rate", type=float, default=50.0)
     ap.add_argument("--seeds", type=int, default=10)

@@ -334,3 +334,115 @@ def serve(args):
                                    f"{n_req} requests, Poisson {args.rate}/s, 80% renders of "
                                    f"log-uniform 1e3-1.92e6 rays, 20% 64^3 SDF/normal tiles of "
                                    f"the 512^3 lattice", "workers": 3, "batch_rays": 131072}}
+
+
+def next_rows(args):
+    """The callers either side of the fine step: the MaskCache ray pre-filter over a
+    1200 x 1600 view (rays/s, against the CPU restatement on a 20k-ray sample), the
+    fine stage's regularisers at 256^3 (smoothed-gradient TV + SDF TV, against the same
+    term written with torch ops on the GPU, the reference's formulation), and mesh
+    extraction (-sdf on a 512^3 lattice + marching cubes)."""
+    import numpy as np
+
+    from oracle import torch_ref as tr   # the CPU baseline of the pre-filter only
+
+    from . import mesh, rays, smooth
+    from ._lib import call, ptr, stream_of
+    from .maskcache import MaskCache
+    from .total_variation_cuda import grid_geom
+    dev = torch.device("cuda:0")
+    out = {"study": "next_rows"}
+    R = 256
+    sdf = scene.sdf_grid(R)
+    # coarse occupancy: the shell |sdf| < 0.1 on a 100^3 density grid
+    x = torch.linspace(-1, 1, 100)
+    X, Y, Z = torch.meshgrid(x, x, x, indexing="ij")
+    shell = ((X * X + Y * Y + Z * Z).sqrt() - 0.5).abs() < 0.1
+    dens = torch.where(shell, 5.0, -20.0)[None, None]
+    mc = MaskCache(density=dens, xyz_min=[-1, -1, -1], xyz_max=[1, 1, 1], act_shift=-4.6,
+                   voxel_size_ratio=1.0).to(dev)
+
+    class Model:
+        pass
+    model = Model()
+    model.mask_cache, model.xyz_min, model.xyz_max, model.voxel_size = mc, mc.xyz_min, mc.xyz_max, 2.0 / R
+    H, W = 1200, 1600
+    K = np.array([[1400.0, 0, W / 2], [0, 1400.0, H / 2], [0, 0, 1]])
+    eye = torch.tensor([0.0, -3.0, 0.5])
+    fwd = -eye / eye.norm()
+    right = torch.linalg.cross(fwd, torch.tensor([0.0, 0.0, 1.0]))
+    right = right / right.norm()
+    up = torch.linalg.cross(right, fwd)
+    c2w = torch.stack([right, up, -fwd, eye], 1)
+    img = torch.rand((H, W, 3), device=dev)
+    kw = {"near": 0.0, "stepsize": 0.5}
+
+    def prefilter():
+        return rays.get_training_rays_in_maskcache_sampling([img], [c2w], [(H, W)], [K], False,
+                                                            False, False, False, model, kw)
+    ms = _event_ms(prefilter, reps=5)
+    kept = prefilter()[4][0]
+    print(f"[next] prefilter {H}x{W}: {ms:.2f} ms, kept {kept}", file=sys.stderr, flush=True)
+    # CPU restatement on a sample of the same view's rays
+    o, d, v = rays.get_rays_of_a_view(H, W, K, c2w.to(dev), False, False, False, False)
+    sel = torch.randperm(H * W, generator=torch.Generator().manual_seed(0))[:20000]
+    oc, dc = o.reshape(-1, 3)[sel.to(dev)].cpu(), d.reshape(-1, 3)[sel.to(dev)].cpu()
+    t0 = time.perf_counter()
+    tr.hit_coarse_geo(oc, dc, [-1.0] * 3, [1.0] * 3, 0.0, 0.5 * 2.0 / R, mc.density.cpu(),
+                      mc.xyz_min.cpu(), mc.xyz_max.cpu(), mc.act_shift, mc.voxel_size_ratio,
+                      mc.mask_cache_thres)
+    cpu_s = time.perf_counter() - t0
+    out["prefilter"] = {"view": [H, W], "ms": round(ms, 3),
+                        "rays_per_s": round(H * W / ms * 1e3, 1), "kept": int(kept),
+                        "cpu_rays_per_s": round(20000 / cpu_s, 1), "cpu_sample_rays": 20000,
+                        "cpu_threads": torch.get_num_threads()}
+    # regularisers at 256^3
+    s = sdf.to(dev)
+    grad = torch.zeros_like(s)
+    lin = torch.linspace(-1, 1, R, device=dev)
+    mask = (lin[:, None, None] ** 2 + lin[None, :, None] ** 2 + lin[None, None, :] ** 2).sqrt() < 0.9
+    cnt = int(mask.sum())
+    ws = smooth.smooth_grad_tv_workspace(s)
+    loss = torch.empty(1, device=dev)
+    geom = grid_geom(s)
+
+    def regularise():
+        smooth.smooth_grad_tv_add_grad(s, grad, 2.0 / R, mask, 5e-4, mask_count=cnt, loss=loss,
+                                       workspace=ws)
+        call("vx_total_variation_add_grad", 0, ptr(s), ptr(grad), geom.data_ptr(), 1e-7, 1e-7,
+             1e-7, 1, stream_of(s))
+    reg_ms = _event_ms(regularise, reps=10)
+    t3 = torch.tensor(smooth.tv_smooth_taps(0.0), dtype=torch.float32)
+    w3 = torch.einsum("i,j,k->ijk", t3, t3, t3)[None, None].to(dev)
+    m5 = mask[None, None]
+
+    def torch_formulation():
+        sg = s.clone().requires_grad_(True)
+        g = torch.zeros([1, 3, R, R, R], device=dev)
+        vs = 2.0 / R
+        g[:, 0, 1:-1] = (sg[:, 0, 2:] - sg[:, 0, :-2]) / 2 / vs
+        g[:, 1, :, 1:-1] = (sg[:, 0, :, 2:] - sg[:, 0, :, :-2]) / 2 / vs
+        g[:, 2, :, :, 1:-1] = (sg[:, 0, :, :, 2:] - sg[:, 0, :, :, :-2]) / 2 / vs
+        gp = g.permute(1, 0, 2, 3, 4)
+        smv = torch.nn.functional.conv3d(torch.nn.functional.pad(gp, (1,) * 6, mode="replicate"), w3)
+        err = (smv.detach() - gp)[m5.repeat(3, 1, 1, 1, 1)] ** 2
+        (err.mean() * 5e-4).backward()
+    torch_ms = _event_ms(torch_formulation, reps=3)
+    print(f"[next] regularisers 256^3: {reg_ms:.3f} ms (torch ops {torch_ms:.3f} ms)",
+          file=sys.stderr, flush=True)
+    out["regularisers_256"] = {"ms": round(reg_ms, 3), "torch_ops_ms": round(torch_ms, 3),
+                               "speedup": round(torch_ms / reg_ms, 1)}
+    # mesh extraction at a 512^3 lattice
+    res = 512
+    bmin, bmax = torch.tensor([-1.0] * 3), torch.tensor([1.0] * 3)
+
+    def extract():
+        u = mesh.extract_fields(s, [-1.0] * 3, [1.0] * 3, bmin, bmax, res)
+        return mesh.marching_cubes(u, 0.0, bmin, bmax)
+    mesh_ms = _event_ms(extract, reps=3)
+    verts, faces = extract()
+    print(f"[next] mesh 512^3: {mesh_ms:.2f} ms, {verts.shape[0]} vertices",
+          file=sys.stderr, flush=True)
+    out["mesh_512"] = {"ms": round(mesh_ms, 2), "vertices": int(verts.shape[0]),
+                       "triangles": int(faces.shape[0])}
+    return out
