@@ -29,6 +29,10 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--prime", type=int, default=256,
+                    help="ray batches whose reachable k0 pages are counted as touched before "
+                         "timing (untimed, no training): the Adam sweep of a short run then "
+                         "covers what a long run's would")
     ap.add_argument("--path", choices=["fused", "ops"], default="fused")
     ap.add_argument("--mlp", choices=["fp32", "bf16"], default="fp32")
     ap.add_argument("--rays", type=int, default=8192)
@@ -105,7 +109,7 @@ def main():
     dev = torch.device("cuda:0")
     mlp_dtype = torch.bfloat16 if args.mlp == "bf16" else torch.float32
 
-    n_pool = 4
+    n_pool = 16
     pool = []
     for i in range(n_pool):
         o, d, v = scene.rays(cfg.n_rays, seed=100 + i)
@@ -126,6 +130,13 @@ def main():
         o, d, v, t = pool[i % n_pool]
         return trainer.step(o, d, v, t, **kw)
 
+    if args.path == "fused" and args.prime > 0:
+        for b0 in range(0, args.prime, 16):
+            batches = []
+            for s in range(b0, min(args.prime, b0 + 16)):
+                o, d, v = scene.rays(cfg.n_rays, seed=10000 + s)
+                batches.append((o.to(dev), d.to(dev), v.to(dev)))
+            trainer.prime_touched(batches)
     for i in range(args.warmup):
         run(i)
     torch.cuda.synchronize()
@@ -159,10 +170,10 @@ def main():
         torch.cuda.synchronize()
         live = trainer.last_live_pages * 1024
         if trainer.overlap_adam:
-            # phase 1: the pages no survivor reaches -- param, exp_avg, exp_avg_sq read
-            # and written, the gradient known zero
-            roof_kernel = "adam_step phase 1 (k0 pages with zero gradient; 256^3 x 12 fp32 grid)"
-            adam_bytes = 24 * (n_k0 - live)
+            # phase 1: the touched pages no survivor reaches this step -- param,
+            # exp_avg, exp_avg_sq read and written, the gradient known zero
+            roof_kernel = "adam_step phase 1 (touched k0 pages with zero gradient; 256^3 x 12 fp32 grid)"
+            adam_bytes = 24 * trainer.last_swept_pages * 1024
         else:
             roof_kernel = "adam_step (k0 grid, 256^3 x 12 fp32, page-flagged gradient)"
             adam_bytes = 24 * n_k0 + 8 * live
@@ -209,6 +220,11 @@ def main():
             "algorithmic_bytes": sb, "achieved_GBs": sb / (ms * 1e-3) / 1e9,
             "frac": sb / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS},
         "samples": counts,
+        "k0_pages": None if args.path != "fused" else {
+            "total": trainer.n_pages, "touched": trainer.touched_pages(),
+            "flagged_last_step": trainer.last_live_pages,
+            "swept_zero_grad_last_step": trainer.last_swept_pages, "primed_batches": args.prime,
+            "sdf_total": trainer._sdf_touched.numel(), "sdf_touched": trainer.sdf_touched_pages()},
         "wall_s": wall,
     }
     if not args.no_cpu:

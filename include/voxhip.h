@@ -292,6 +292,14 @@ int vx_adam_upd(int dtype, void* param, const void* grad, void* exp_avg, void* e
 int vx_adam_step(int dtype, void* param, void* grad, void* exp_avg, void* exp_avg_sq,
                  const void* per_lr, int64_t n, int step, double lr, double beta1, double beta2,
                  double eps, int clear_grad, uint32_t* pages, void* stream);
+/* lib/utils.py:169-199 utils.Adam on one float32 parameter (clear_grad = 1, 16-byte
+ * aligned buffers), skipping pages of 1024 elements whose update is the identity:
+ * touched word zero (moments still zero) and an all-zero gradient this step.  A page with
+ * a nonzero gradient is marked touched.  touched: ceil(n / 1024) words, zero-initialised
+ * with the moments (all ones after loading moments from elsewhere). */
+int vx_adam_step_touched(float* param, float* grad, float* exp_avg, float* exp_avg_sq,
+                         const float* per_lr, int64_t n, int step, double lr, double beta1,
+                         double beta2, double eps, uint32_t* touched, void* stream);
 /* vx_adam_step (float32, clear_grad = 1, 16-byte aligned buffers) split in two passes
  * over disjoint element sets, for overlap with other work.  pages holds
  * 2 * ceil(n / 1024) + 1 words: the page flags, a count and a page list (count and list
@@ -299,10 +307,14 @@ int vx_adam_step(int dtype, void* param, void* grad, void* exp_avg, void* exp_av
  * it may run as soon as the flags are final, before the gradient is) from one block per
  * CU, and lists the flagged pages; phase 2 updates the listed pages, clears their
  * gradient and resets flags and count.  Phase 1 then phase 2 with the same step equal
- * one vx_adam_step. */
+ * one vx_adam_step.  touched (ceil(n / 1024) words, or NULL): nonzero for every page whose
+ * moments may be nonzero; phase 1 skips pages with a zero flag and a zero touched word
+ * (their p, m and v are left exactly as utils.Adam leaves them: m = v = g = 0 makes the
+ * update an identity) and phase 2 marks the pages it updates.  Zero-initialise it with
+ * the moments; set it to all ones after loading moments from elsewhere. */
 int vx_adam_step_split(int phase, float* param, float* grad, float* exp_avg, float* exp_avg_sq,
                        const float* per_lr, int64_t n, int step, double lr, double beta1,
-                       double beta2, double eps, uint32_t* pages, void* stream);
+                       double beta2, double eps, uint32_t* pages, uint32_t* touched, void* stream);
 
 /* ---- per-sample ops ------------------------------------------------------------ */
 

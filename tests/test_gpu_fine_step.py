@@ -156,3 +156,37 @@ def test_split_k0_update_over_consecutive_steps(dev, setup):
         p = t.k0.detach().cpu().double()
         m = t.adam["k0"][0].detach().cpu().double()
         vv = t.adam["k0"][1].detach().cpu().double()
+
+
+def test_touched_page_skip_over_changing_batches(dev, setup):
+    """Three steps on different ray batches: pages touched by an earlier step but not
+    flagged now are swept with their decaying moments, pages never touched are skipped
+    (their update is the identity); after every step k0 and its moments equal utils.Adam
+    applied in order to the whole grid."""
+    st, _, _, _ = setup
+    t = FusedFineTrainer(CFG, st["sdf"], st["k0"], st["mlp"], device=dev)
+    p = t.k0.detach().cpu().double()
+    m = torch.zeros_like(p)
+    vv = torch.zeros_like(p)
+    for i in range(3):
+        o, d, v = (x.to(dev) for x in scene.rays(CFG.n_rays, seed=50 + i))
+        t.step(o, d, v, scene.targets(CFG.n_rays, seed=50 + i).to(dev), profile=True)
+        g = t.grads_snapshot["k0"].cpu().double()
+        tr.adam_step(p, g, m, vv, i + 1, CFG.lr_k0, 0.9, 0.99, 1e-8)
+        close(t.k0, p, 1e-5, 1e-7, f"k0 after step {i + 1}")
+        close(t.adam["k0"][0], m, 1e-5, 1e-7, f"k0 exp_avg after step {i + 1}")
+        close(t.adam["k0"][1], vv, 1e-5, 1e-7, f"k0 exp_avg_sq after step {i + 1}")
+        if i > 0:
+            assert t.last_swept_pages > 0   # touched earlier, not flagged now
+        p = t.k0.detach().cpu().double()
+        m = t.adam["k0"][0].detach().cpu().double()
+        vv = t.adam["k0"][1].detach().cpu().double()
+    assert 0 < t.touched_pages() < t.n_pages
+    # untouched pages: parameters exactly as initialised, moments exactly zero
+    untouched = (t._touched == 0).cpu()
+    flat = lambda x: x.detach().permute(0, 2, 3, 4, 1).reshape(-1).cpu()  # noqa: E731
+    k0_init = st["k0"].permute(0, 2, 3, 4, 1).reshape(-1)
+    n = k0_init.numel()
+    sel = untouched.repeat_interleave(1024)[:n]
+    assert torch.equal(flat(t.k0)[sel], k0_init[sel])
+    assert not bool(flat(t.adam["k0"][0])[sel].any()) and not bool(flat(t.adam["k0"][1])[sel].any())

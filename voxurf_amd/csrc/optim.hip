@@ -125,15 +125,24 @@ __global__ void __launch_bounds__(256) k_adam_step_f32(float* __restrict__ p, fl
 }
 
 // The two halves of vx_adam_step_split.  `pages` holds [flags (npages) | count | list
-// (npages)]: phase 1 updates the pages whose flag is zero and appends every flagged
-// page to the list; phase 2 updates the listed pages.  A page is 1024 elements = 256
-// float4, one per thread of a 256-thread block.
+// (npages)]: phase 1 updates the pages whose flag is zero (and that were touched) and
+// appends every flagged page to the list; phase 2 updates the listed pages.  A page is
+// 1024 elements = 256 float4, one per thread of a 256-thread block.
+//
+// Phase 1 takes the pages 256 at a time: each thread classifies one page (one load of
+// its flag and touched word), the block compacts the pages to sweep into LDS and appends
+// the flagged ones to the list with one atomic, then sweeps its pages U at a time with
+// every load issued before the first store.
 template <int U>
 __global__ void __launch_bounds__(256) k_adam_dead_pages(float* __restrict__ p, float* __restrict__ m,
                                                          float* __restrict__ v,
                                                          const float* __restrict__ perlr, int64_t n,
                                                          AdamCoef c, uint32_t* __restrict__ pages,
+                                                         const uint32_t* __restrict__ touched,
                                                          int64_t npages) {
+  __shared__ uint32_t sweep[256];
+  __shared__ uint32_t wave_live[4], wave_dead[4];
+  __shared__ uint32_t live_base;
   uint32_t* count = pages + npages;
   uint32_t* list = pages + npages + 1;
   const int64_t n4 = n >> 2;
@@ -141,49 +150,72 @@ __global__ void __launch_bounds__(256) k_adam_dead_pages(float* __restrict__ p, 
   float4* m4 = reinterpret_cast<float4*>(m);
   float4* v4 = reinterpret_cast<float4*>(v);
   const float4* l4 = reinterpret_cast<const float4*>(perlr);
-  const int tid = threadIdx.x;
-  for (int64_t pg0 = (int64_t)blockIdx.x * U; pg0 < npages; pg0 += (int64_t)gridDim.x * U) {
-    bool dead[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) dead[u] = pg0 + u < npages && pages[pg0 + u] == 0u;
-    if (tid == 0) {
-#pragma unroll
-      for (int u = 0; u < U; ++u)
-        if (pg0 + u < npages && !dead[u]) list[atomicAdd(count, 1u)] = (uint32_t)(pg0 + u);
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  for (int64_t c0 = (int64_t)blockIdx.x * 256; c0 < npages; c0 += (int64_t)gridDim.x * 256) {
+    // 1. classify: flagged -> listed for phase 2; unflagged and touched -> swept here (a
+    // page never touched has m = v = 0, so with g = 0 its update is the identity)
+    const int64_t pg = c0 + tid;
+    const bool in = pg < npages;
+    const bool live = in && pages[pg] != 0u;
+    const bool dead = in && !live && (!touched || touched[pg] != 0u);
+    const uint64_t lb = __ballot(live), db = __ballot(dead);
+    if (lane == 0) {
+      wave_live[w] = (uint32_t)__popcll(lb);
+      wave_dead[w] = (uint32_t)__popcll(db);
     }
-    float4 pp[U], mm[U], vv[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int64_t i = (pg0 + u) * 256 + tid;
-      if (dead[u] && i < n4) {
-        pp[u] = ldnt(p4 + i);
-        mm[u] = ldnt(m4 + i);
-        vv[u] = ldnt(v4 + i);
+    __syncthreads();
+    uint32_t lo = 0, dofs = 0, ndead = 0, nlive = 0;
+    for (int k = 0; k < 4; ++k) {
+      if (k < w) {
+        lo += wave_live[k];
+        dofs += wave_dead[k];
       }
+      nlive += wave_live[k];
+      ndead += wave_dead[k];
     }
+    if (tid == 0 && nlive) live_base = atomicAdd(count, nlive);
+    if (dead) sweep[dofs + (uint32_t)__popcll(db & below)] = (uint32_t)pg;
+    __syncthreads();
+    if (live) list[live_base + lo + (uint32_t)__popcll(lb & below)] = (uint32_t)pg;
+    // 2. sweep this chunk's pages, U at a time
+    for (uint32_t k0 = 0; k0 < ndead; k0 += U) {
+      float4 pp[U], mm[U], vv[U];
+      int64_t idx[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int64_t i = (pg0 + u) * 256 + tid;
-      if (dead[u] && i < n4) {
-        const float4 ll = perlr ? l4[i] : make_float4(1.f, 1.f, 1.f, 1.f);
-        adam_elem(pp[u].x, 0.f, mm[u].x, vv[u].x, ll.x, c);
-        adam_elem(pp[u].y, 0.f, mm[u].y, vv[u].y, ll.y, c);
-        adam_elem(pp[u].z, 0.f, mm[u].z, vv[u].z, ll.z, c);
-        adam_elem(pp[u].w, 0.f, mm[u].w, vv[u].w, ll.w, c);
-        stnt(p4 + i, pp[u]);
-        stnt(m4 + i, mm[u]);
-        stnt(v4 + i, vv[u]);
-      } else if (dead[u] && i == n4) {
-        // the n % 4 trailing elements (last page)
-        for (int64_t e = n4 << 2; e < n; ++e) {
-          float a = p[e], b = m[e], d = v[e];
-          adam_elem(a, 0.f, b, d, perlr ? perlr[e] : 1.f, c);
-          p[e] = a;
-          m[e] = b;
-          v[e] = d;
+      for (int u = 0; u < U; ++u) {
+        idx[u] = k0 + u < ndead ? (int64_t)sweep[k0 + u] * 256 + tid : -1;
+        if (idx[u] >= 0 && idx[u] < n4) {
+          pp[u] = ldnt(p4 + idx[u]);
+          mm[u] = ldnt(m4 + idx[u]);
+          vv[u] = ldnt(v4 + idx[u]);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int64_t i = idx[u];
+        if (i >= 0 && i < n4) {
+          const float4 ll = perlr ? l4[i] : make_float4(1.f, 1.f, 1.f, 1.f);
+          adam_elem(pp[u].x, 0.f, mm[u].x, vv[u].x, ll.x, c);
+          adam_elem(pp[u].y, 0.f, mm[u].y, vv[u].y, ll.y, c);
+          adam_elem(pp[u].z, 0.f, mm[u].z, vv[u].z, ll.z, c);
+          adam_elem(pp[u].w, 0.f, mm[u].w, vv[u].w, ll.w, c);
+          stnt(p4 + i, pp[u]);
+          stnt(m4 + i, mm[u]);
+          stnt(v4 + i, vv[u]);
+        } else if (i == n4) {
+          // the n % 4 trailing elements (last page)
+          for (int64_t e = n4 << 2; e < n; ++e) {
+            float a = p[e], b = m[e], d = v[e];
+            adam_elem(a, 0.f, b, d, perlr ? perlr[e] : 1.f, c);
+            p[e] = a;
+            m[e] = b;
+            v[e] = d;
+          }
         }
       }
     }
+    __syncthreads();  // sweep[] and the wave counts are reused by the next chunk
   }
 }
 
@@ -191,6 +223,7 @@ __global__ void __launch_bounds__(256) k_adam_live_pages(float* __restrict__ p, 
                                                          float* __restrict__ m, float* __restrict__ v,
                                                          const float* __restrict__ perlr, int64_t n,
                                                          AdamCoef c, const uint32_t* __restrict__ pages,
+                                                         uint32_t* __restrict__ touched,
                                                          int64_t npages) {
   const uint32_t cnt = pages[npages];
   const uint32_t* list = pages + npages + 1;
@@ -209,7 +242,10 @@ __global__ void __launch_bounds__(256) k_adam_live_pages(float* __restrict__ p, 
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       idx[u] = -1;
-      if (k0 + u < (int64_t)cnt) idx[u] = (int64_t)list[k0 + u] * 256 + tid;
+      if (k0 + u < (int64_t)cnt) {
+        idx[u] = (int64_t)list[k0 + u] * 256 + tid;
+        if (touched && tid == 0) touched[list[k0 + u]] = 1u;
+      }
       if (idx[u] >= 0 && idx[u] < n4) {
         pp[u] = ldnt(p4 + idx[u]);
         gg[u] = ldnt(g4 + idx[u]);
@@ -264,6 +300,82 @@ __global__ void k_adam_step_f64(double* __restrict__ p, double* __restrict__ g,
     const double denom = sqrt(vi) / sqrt_bc2 + eps;
     p[i] = p[i] + neg_step * ((mi * (perlr ? perlr[i] : 1.0)) / denom);
     if (clear) g[i] = 0;
+  }
+}
+
+// Dense utils.Adam pass that skips pages whose update is the identity: a page never
+// touched before (touched word zero: its moments are still zero) with an all-zero
+// gradient this step.  Each block takes U pages per iteration; the gradient of a page
+// is read first, the page's parameters and moments only if it is touched or its
+// gradient is nonzero (which marks it touched from now on).
+template <int U>
+__global__ void __launch_bounds__(256) k_adam_touched(float* __restrict__ p, float* __restrict__ g,
+                                                      float* __restrict__ m, float* __restrict__ v,
+                                                      const float* __restrict__ perlr, int64_t n,
+                                                      AdamCoef c, uint32_t* __restrict__ touched,
+                                                      int64_t npages) {
+  const int64_t n4 = n >> 2;
+  float4* p4 = reinterpret_cast<float4*>(p);
+  float4* g4 = reinterpret_cast<float4*>(g);
+  float4* m4 = reinterpret_cast<float4*>(m);
+  float4* v4 = reinterpret_cast<float4*>(v);
+  const float4* l4 = reinterpret_cast<const float4*>(perlr);
+  const int tid = threadIdx.x;
+  for (int64_t pg0 = (int64_t)blockIdx.x * U; pg0 < npages; pg0 += (int64_t)gridDim.x * U) {
+    float4 gg[U];
+    bool was[U], nz[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t i = (pg0 + u) * 256 + tid;
+      was[u] = pg0 + u < npages && touched[pg0 + u] != 0u;
+      nz[u] = false;
+      gg[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (pg0 + u < npages && i < n4) {
+        gg[u] = ldnt(g4 + i);
+        nz[u] = gg[u].x != 0.f || gg[u].y != 0.f || gg[u].z != 0.f || gg[u].w != 0.f;
+      } else if (pg0 + u < npages && i == n4) {
+        for (int64_t e = n4 << 2; e < n; ++e) nz[u] = nz[u] || g[e] != 0.f;
+      }
+    }
+    bool any[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) any[u] = __syncthreads_or(nz[u]) != 0;
+    float4 pp[U], mm[U], vv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t i = (pg0 + u) * 256 + tid;
+      if ((was[u] || any[u]) && pg0 + u < npages && i < n4) {
+        pp[u] = ldnt(p4 + i);
+        mm[u] = ldnt(m4 + i);
+        vv[u] = ldnt(v4 + i);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (!(was[u] || any[u]) || pg0 + u >= npages) continue;
+      const int64_t i = (pg0 + u) * 256 + tid;
+      if (i < n4) {
+        const float4 ll = perlr ? l4[i] : make_float4(1.f, 1.f, 1.f, 1.f);
+        adam_elem(pp[u].x, gg[u].x, mm[u].x, vv[u].x, ll.x, c);
+        adam_elem(pp[u].y, gg[u].y, mm[u].y, vv[u].y, ll.y, c);
+        adam_elem(pp[u].z, gg[u].z, mm[u].z, vv[u].z, ll.z, c);
+        adam_elem(pp[u].w, gg[u].w, mm[u].w, vv[u].w, ll.w, c);
+        stnt(p4 + i, pp[u]);
+        stnt(m4 + i, mm[u]);
+        stnt(v4 + i, vv[u]);
+        if (any[u]) stnt(g4 + i, make_float4(0.f, 0.f, 0.f, 0.f));
+      } else if (i == n4) {
+        for (int64_t e = n4 << 2; e < n; ++e) {
+          float a = p[e], b = m[e], d = v[e];
+          adam_elem(a, g[e], b, d, perlr ? perlr[e] : 1.f, c);
+          p[e] = a;
+          m[e] = b;
+          v[e] = d;
+          g[e] = 0.f;
+        }
+      }
+      if (any[u] && !was[u] && tid == 0) touched[pg0 + u] = 1u;
+    }
   }
 }
 
@@ -342,9 +454,42 @@ int vx_adam_step(int dtype, void* param, void* grad, void* exp_avg, void* exp_av
   return launch_status("adam_step");
 }
 
+int vx_adam_step_touched(float* param, float* grad, float* exp_avg, float* exp_avg_sq,
+                         const float* per_lr, int64_t n, int step, double lr, double beta1,
+                         double beta2, double eps, uint32_t* touched, void* stream) {
+  VX_CHECK_ARG(n >= 0, "adam_step_touched: n < 0");
+  VX_CHECK_ARG(touched != nullptr, "adam_step_touched: touched words required");
+  VX_CHECK_ARG(step >= 1, "adam_step_touched: step must be >= 1");
+  VX_CHECK_ARG((((uintptr_t)param | (uintptr_t)grad | (uintptr_t)exp_avg | (uintptr_t)exp_avg_sq |
+                 (uintptr_t)per_lr) % 16) == 0,
+               "adam_step_touched: buffers must be 16-byte aligned");
+  if (n == 0) return VX_OK;
+  const double bc1 = 1.0 - pow(beta1, (double)step);
+  const double bc2 = 1.0 - pow(beta2, (double)step);
+  AdamCoef c;
+  c.b1 = (float)beta1;
+  c.omb1 = (float)(1.0 - beta1);
+  c.b2 = (float)beta2;
+  c.omb2 = (float)(1.0 - beta2);
+  c.sqrt_bc2 = (float)sqrt(bc2);
+  c.eps = (float)eps;
+  c.neg_step = (float)(-(lr / bc1));
+  const int64_t np = (n + (1 << kPageShift) - 1) >> kPageShift;
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+    cus = 256;
+  constexpr int U = 4;
+  const int64_t blocks = std::min<int64_t>(8 * (int64_t)cus, cdiv(np, U));
+  k_adam_touched<U><<<(unsigned)blocks, 256, 0, (hipStream_t)stream>>>(
+      param, grad, exp_avg, exp_avg_sq, per_lr, n, c, touched, np);
+  return launch_status("adam_step_touched");
+}
+
 int vx_adam_step_split(int phase, float* param, float* grad, float* exp_avg, float* exp_avg_sq,
                        const float* per_lr, int64_t n, int step, double lr, double beta1,
-                       double beta2, double eps, uint32_t* pages, void* stream) {
+                       double beta2, double eps, uint32_t* pages, uint32_t* touched,
+                       void* stream) {
   VX_CHECK_ARG(phase == 1 || phase == 2, "adam_step_split: phase must be 1 or 2");
   VX_CHECK_ARG(pages != nullptr, "adam_step_split: page flags required");
   VX_CHECK_ARG(n >= 0, "adam_step_split: n < 0");
@@ -376,13 +521,13 @@ int vx_adam_step_split(int phase, float* param, float* grad, float* exp_avg, flo
     // (4 pages per iteration keeps it under 128 VGPRs, so an rgbnet layer block -- two
     // waves of ~176 VGPRs per SIMD -- still fits beside it)
     constexpr int U = 4;
-    const int64_t blocks = std::min<int64_t>(cus, cdiv(np, U));
+    const int64_t blocks = std::min<int64_t>(cus, cdiv(np, 256));
     k_adam_dead_pages<U><<<(unsigned)blocks, 256, 0, st>>>(param, exp_avg, exp_avg_sq, per_lr, n, c,
-                                                          pages, np);
+                                                          pages, touched, np);
   } else {
     const int64_t blocks = std::min<int64_t>(4 * (int64_t)cus, cdiv(np, 4));
     k_adam_live_pages<<<(unsigned)blocks, 256, 0, st>>>(param, grad, exp_avg, exp_avg_sq, per_lr, n,
-                                                        c, pages, np);
+                                                        c, pages, touched, np);
     // reset the flags and the list count for the next step
     k_zero_u32<<<grid_for(np + 1, 256), 256, 0, st>>>(pages, np + 1);
   }

@@ -77,6 +77,14 @@ class FusedFineTrainer:
                        for _ in range(2)]
         self._parity = 0
         self.n_pages = n_pages
+        # One word per k0 page, nonzero once any step has flagged the page: the split
+        # Adam skips unflagged pages that were never touched, whose moments are still
+        # zero (the update is then the identity, as in utils.Adam).
+        self._touched = torch.zeros(n_pages, dtype=torch.int32, device=self.dev)
+        # the same for the SDF grid, whose one-pass update skips untouched pages with an
+        # all-zero gradient (vx_adam_step_touched)
+        self._sdf_touched = torch.zeros((self.sdf.numel() + 1023) // 1024, dtype=torch.int32,
+                                        device=self.dev)
         self.adam = {
             "sdf": [torch.zeros_like(self.sdf), torch.zeros_like(self.sdf), 0, cfg.lr_sdf],
             "k0": [torch.zeros_like(self._k0), torch.zeros_like(self._k0), 0, cfg.lr_k0],
@@ -164,7 +172,7 @@ class FusedFineTrainer:
         self.loss_r = torch.empty(n, dtype=torch.float32, device=d)
         self._n = n
 
-    def _adam(self, name, p, g, pages=None, phase=0, step=None, timed=False):
+    def _adam(self, name, p, g, pages=None, phase=0, step=None, timed=False, touched=None):
         """utils.Adam on one grid (betas (0.9, 0.99), eps 1e-8), clearing its gradient.
         phase 1/2: the two halves of vx_adam_step_split (pages required); `step` is the
         step count both halves share."""
@@ -177,12 +185,15 @@ class FusedFineTrainer:
         if timed:
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record()
-        if phase == 0:
+        if phase == 0 and touched is not None:
+            call("vx_adam_step_touched", ptr(p), ptr(g), ptr(m), ptr(v), None, p.numel(), step,
+                 float(lr), 0.9, 0.99, 1e-8, ptr(touched), st)
+        elif phase == 0:
             call("vx_adam_step", 0, ptr(p), ptr(g), ptr(m), ptr(v), None, p.numel(), step,
                  float(lr), 0.9, 0.99, 1e-8, 1, ptr(pages), st)
         else:
             call("vx_adam_step_split", phase, ptr(p), ptr(g), ptr(m), ptr(v), None, p.numel(),
-                 step, float(lr), 0.9, 0.99, 1e-8, ptr(pages), st)
+                 step, float(lr), 0.9, 0.99, 1e-8, ptr(pages), ptr(self._touched), st)
         if timed:
             e1.record()
             self.adam_events.append((e0, e1))
@@ -273,7 +284,11 @@ class FusedFineTrainer:
         if profile:
             # snapshot before any optimiser pass clears a gradient
             k0_grad_snapshot = self.k0_grad.clone()
-            self.last_live_pages = int(pages[:self.n_pages].count_nonzero())
+            flags = pages[:self.n_pages] != 0
+            self.last_live_pages = int(flags.sum())
+            # pages phase 1 swept this step: touched before it and not flagged (phase 2
+            # of this step has not run yet, so touched is still the state phase 1 read)
+            self.last_swept_pages = int(((self._touched != 0) & ~flags).sum())
         if self.overlap_adam:
             # the flagged pages, once the feature scatter has finished
             grads_ready = torch.cuda.Event()
@@ -286,6 +301,9 @@ class FusedFineTrainer:
         else:
             self._adam("k0", self._k0, self.k0_grad, pages=pages, phase=0, step=k0_step,
                        timed=True)
+            # the one-pass update does not record touched pages: count every page as
+            # touched from now on (skipping is then off, which is always exact)
+            self._touched.fill_(1)
         call("vx_fused_fine_bwd", ptr(self.sdf), self.sdf_geom.data_ptr(), self.box.data_ptr(),
              self.params.data_ptr(), ptr(viewdirs), n, self.cap, ptr(self.s4), ptr(self.g4),
              ptr(self.stats), ptr(self.cnt), ptr(self.off), ptr(self.last), ptr(gw),
@@ -294,7 +312,7 @@ class FusedFineTrainer:
         if profile:
             self.grads_snapshot = dict(sdf=self.sdf_grad.clone(), k0=k0_grad_snapshot,
                                        mlp=[g.clone() for g in self.mlp_grads])
-        self._adam("sdf", self.sdf, self.sdf_grad)
+        self._adam("sdf", self.sdf, self.sdf_grad, touched=self._sdf_touched)
         self._adam("mlp", self.mlp_flat, self.mlp_gflat)
         # mean squared error over the batch's N x 3 colour components (run.py:604)
         loss = torch.empty(1, dtype=torch.float32, device=self.dev)
@@ -335,6 +353,46 @@ class FusedFineTrainer:
         return out
 
     @torch.no_grad()
+    def prime_touched(self, batches):
+        """Count as touched every k0 page the survivors of the given ray batches
+        ((rays_o, rays_d, viewdirs) each) can reach, and every SDF page their walk-range
+        samples' stencils can reach, without training on them.  Marking pages is always
+        exact (a touched page is simply updated); this sizes the sweeps of a short run to
+        those of a long one, whose rays have visited the scene widely."""
+        self.k0  # waits for a pending k0 update
+        flags = torch.zeros(2 * self.n_pages + 1, dtype=torch.int32, device=self.dev)
+        for rays_o, rays_d, viewdirs in batches:
+            n = rays_o.shape[0]
+            self._ensure(n)
+            call("vx_fused_fine_fwd", ptr(self.sdf), self.sdf_geom.data_ptr(), self.box.data_ptr(),
+                 self.params.data_ptr(), *self._mc_args(), ptr(rays_o), ptr(rays_d),
+                 ptr(viewdirs), n, self.cap, ptr(self.s4), ptr(self.g4), ptr(self.cnt),
+                 ptr(self.last), ptr(self.stats), 0, ptr(self.overflow), self.k0_geom.data_ptr(),
+                 ptr(flags), stream_of(self.sdf))
+            # SDF pages: the 7-point stencils (one voxel around the sample's cell) of the
+            # walk-range samples, the only samples with an SDF gradient
+            valid = (torch.arange(self.cap, device=self.dev)[None, :] <
+                     self.stats[:n, 1:2].long()).reshape(-1)
+            q = self.s4[:n * self.cap][valid, :3]
+            R = torch.tensor(self.sdf.shape[-3:], device=self.dev)
+            idx = ((q + 1.0) / 2.0 * (R - 1).float()).floor().long()
+            off = torch.stack(torch.meshgrid(*[torch.arange(-1, 3, device=self.dev)] * 3,
+                                             indexing="ij"), -1).reshape(-1, 3)
+            cells = (idx[:, None, :] + off[None]).reshape(-1, 3)
+            cells = torch.minimum(torch.maximum(cells, torch.zeros_like(R)), R - 1)
+            lin = (cells[:, 0] * R[1] + cells[:, 1]) * R[2] + cells[:, 2]
+            self._sdf_touched[torch.unique(lin // 1024)] = 1
+        self._touched |= (flags[:self.n_pages] != 0).to(torch.int32)
+
+    def touched_pages(self):
+        """Number of k0 pages counted as touched."""
+        return int(self._touched.count_nonzero())
+
+    def sdf_touched_pages(self):
+        """Number of SDF-grid pages counted as touched."""
+        return int(self._sdf_touched.count_nonzero())
+
+    @torch.no_grad()
     def load_grids(self, sdf, k0):
         """Replace the SDF and feature grids (same shapes) in place."""
         k0_dst = self.k0  # waits for a pending k0 update
@@ -368,6 +426,8 @@ class FusedFineTrainer:
         if off != m.numel():
             raise ValueError("optimizer state: rgbnet moments do not match the network")
         self.adam["mlp"][2] = step
+        self._touched.fill_(1)  # loaded moments may be nonzero anywhere
+        self._sdf_touched.fill_(1)
 
     def state_dict(self):
         sd = {"sdf.grid": self.sdf, "k0.grid": self.k0}
