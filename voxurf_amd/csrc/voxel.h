@@ -118,20 +118,42 @@ __device__ __forceinline__ bool in_grid(int x, int y, int z, const GridDesc& g) 
   return (unsigned)x < (unsigned)g.X && (unsigned)y < (unsigned)g.Y && (unsigned)z < (unsigned)g.Z;
 }
 
-// Gather one channel: sum over corners in grid_sample's order (x outer, z inner).
+__device__ __forceinline__ int clamp_index(int v, int n) { return v < 0 ? 0 : (v >= n ? n - 1 : v); }
+
+// Gather one channel: sum over corners in grid_sample's order (x outer, z inner),
+// corners outside the grid skipped (zero padding).  The eight loads are unconditional
+// (addresses clamped into the grid) so they are all in flight together; the sum then
+// selects the in-grid terms, with the same operations in the same order.
+template <typename T>
+__device__ __forceinline__ void tri_load(const T* __restrict__ grid, const GridDesc& g,
+                                         const Tri<T>& t, int c, T v[8], unsigned& in) {
+  in = 0u;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int x = t.x0 + (k >> 2), y = t.y0 + (k >> 1 & 1), z = t.z0 + (k & 1);
+    if (in_grid<T>(x, y, z, g)) in |= 1u << k;
+    v[k] = grid[c * g.sc + clamp_index(x, g.X) * g.sx + clamp_index(y, g.Y) * g.sy +
+                clamp_index(z, g.Z) * g.sz];
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ T tri_sum(const Tri<T>& t, const T v[8], unsigned in) {
+#pragma clang fp contract(off)
+  T acc = 0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    if (in >> k & 1u) acc += v[k] * t.w(k >> 2, k >> 1 & 1, k & 1);
+  return acc;
+}
+
 template <typename T>
 __device__ __forceinline__ T tri_gather(const T* __restrict__ grid, const GridDesc& g,
                                         const Tri<T>& t, int c) {
-#pragma clang fp contract(off)
-  T acc = 0;
-  for (int bx = 0; bx < 2; ++bx)
-    for (int by = 0; by < 2; ++by)
-      for (int bz = 0; bz < 2; ++bz) {
-        const int x = t.x0 + bx, y = t.y0 + by, z = t.z0 + bz;
-        if (in_grid<T>(x, y, z, g))
-          acc += grid[c * g.sc + x * g.sx + y * g.sy + z * g.sz] * t.w(bx, by, bz);
-      }
-  return acc;
+  T v[8];
+  unsigned in;
+  tri_load(grid, g, t, c, v, in);
+  return tri_sum(t, v, in);
 }
 
 // Four consecutive channels c0 .. c0+3 of a channels-innermost grid (channel stride 1,
@@ -140,20 +162,25 @@ __device__ __forceinline__ T tri_gather(const T* __restrict__ grid, const GridDe
 __device__ __forceinline__ float4 tri_gather4(const float* __restrict__ grid, const GridDesc& g,
                                              const Tri<float>& t, int c0) {
 #pragma clang fp contract(off)
+  float4 v[8];
+  bool in[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int x = t.x0 + (k >> 2), y = t.y0 + (k >> 1 & 1), z = t.z0 + (k & 1);
+    in[k] = in_grid<float>(x, y, z, g);
+    v[k] = *reinterpret_cast<const float4*>(grid + c0 + clamp_index(x, g.X) * g.sx +
+                                            clamp_index(y, g.Y) * g.sy + clamp_index(z, g.Z) * g.sz);
+  }
   float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-  for (int bx = 0; bx < 2; ++bx)
-    for (int by = 0; by < 2; ++by)
-      for (int bz = 0; bz < 2; ++bz) {
-        const int x = t.x0 + bx, y = t.y0 + by, z = t.z0 + bz;
-        if (in_grid<float>(x, y, z, g)) {
-          const float4 v = *reinterpret_cast<const float4*>(grid + c0 + x * g.sx + y * g.sy + z * g.sz);
-          const float w = t.w(bx, by, bz);
-          acc.x += v.x * w;
-          acc.y += v.y * w;
-          acc.z += v.z * w;
-          acc.w += v.w * w;
-        }
-      }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    if (!in[k]) continue;
+    const float w = t.w(k >> 2, k >> 1 & 1, k & 1);
+    acc.x += v[k].x * w;
+    acc.y += v[k].y * w;
+    acc.z += v[k].z * w;
+    acc.w += v[k].w * w;
+  }
   return acc;
 }
 
@@ -317,31 +344,6 @@ __device__ __forceinline__ void neus_bwd(const T* dir, const NeusFwd<T>& f, T di
 // sdf at the point, grad[3] in xyz order (unnormalised), optional feat[6]
 // (-x,+x,-y,+y,-z,+z).
 
-template <typename T>
-__device__ __forceinline__ void sdf7(const T* __restrict__ grid, const GridDesc& g, const T q[3],
-                                     const T* mn, const T* mx, T voxel_size, T& sdf, T grad[3],
-                                     T* feat) {
-#pragma clang fp contract(off)
-  Tri<T> t;
-  t.init(world_to_index(q[0], mn[0], mx[0] - mn[0], g.X),
-         world_to_index(q[1], mn[1], mx[1] - mn[1], g.Y),
-         world_to_index(q[2], mn[2], mx[2] - mn[2], g.Z));
-  sdf = tri_gather(grid, g, t, 0);
-  T ind[3];
-  zyx_index(q, mn, mx, g, ind);
-  for (int a = 0; a < 3; ++a) {
-    T cm, cp;
-    const T fm = sample_displaced(grid, g, ind, a, (T)-1, cm);
-    const T fp = sample_displaced(grid, g, ind, a, (T)1, cp);
-    const T df = clamped_diff(ind, a, (T)1, g);
-    grad[2 - a] = ((fp - fm) / df) / voxel_size;
-    if (feat) {
-      feat[2 * (2 - a)] = fm;
-      feat[2 * (2 - a) + 1] = fp;
-    }
-  }
-}
-
 // Trilinear cell of the single-channel grid at the zyx index displaced along `axis`
 // (clamped), as sample_displaced / scatter_displaced compute it.
 template <typename T>
@@ -356,6 +358,42 @@ __device__ __forceinline__ void displaced_cell(const GridDesc& g, const T ind[3]
     pos[k] = roundtrip_index(a, sz[k]);
   }
   t.init(pos[2], pos[1], pos[0]);
+}
+
+template <typename T>
+__device__ __forceinline__ void sdf7(const T* __restrict__ grid, const GridDesc& g, const T q[3],
+                                     const T* mn, const T* mx, T voxel_size, T& sdf, T grad[3],
+                                     T* feat) {
+#pragma clang fp contract(off)
+  // the seven cells first, then all 56 corner loads, then the sums: one memory latency
+  // per sample instead of seven
+  Tri<T> t[7];
+  t[0].init(world_to_index(q[0], mn[0], mx[0] - mn[0], g.X),
+            world_to_index(q[1], mn[1], mx[1] - mn[1], g.Y),
+            world_to_index(q[2], mn[2], mx[2] - mn[2], g.Z));
+  T ind[3];
+  zyx_index(q, mn, mx, g, ind);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    displaced_cell(g, ind, a, (T)-1, t[1 + 2 * a]);
+    displaced_cell(g, ind, a, (T)1, t[2 + 2 * a]);
+  }
+  T v[7][8];
+  unsigned in[7];
+#pragma unroll
+  for (int i = 0; i < 7; ++i) tri_load(grid, g, t[i], 0, v[i], in[i]);
+  sdf = tri_sum(t[0], v[0], in[0]);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const T fm = tri_sum(t[1 + 2 * a], v[1 + 2 * a], in[1 + 2 * a]);
+    const T fp = tri_sum(t[2 + 2 * a], v[2 + 2 * a], in[2 + 2 * a]);
+    const T df = clamped_diff(ind, a, (T)1, g);
+    grad[2 - a] = ((fp - fm) / df) / voxel_size;
+    if (feat) {
+      feat[2 * (2 - a)] = fm;
+      feat[2 * (2 - a) + 1] = fp;
+    }
+  }
 }
 
 // Adjoint of sdf7: ggrid += d(sdf, grad, feat)^T (gsdf, ggrad, gfeat); gfeat may be null.
