@@ -127,6 +127,160 @@ __global__ void k_pack_net(PackNet net, void* __restrict__ out) {
 // float32.  Persistent: each wave walks 32-row tiles; weights stay in LDS.
 constexpr int kLayerThreads = 512;
 
+// One 32-row tile of a layer, output column tiles [to_lo, to_hi) of the NTO (FULL: all).
+template <int KT, int NTO, int MODE, bool BF, bool FULL>
+__device__ __forceinline__ void layer_tile(const void* __restrict__ in_, int in_stride, int64_t M,
+                                           const unsigned char* lds_raw, const float* lb,
+                                           const void* __restrict__ mask_, int mask_stride,
+                                           void* __restrict__ out_, int out_stride, int64_t tile,
+                                           int to_lo, int to_hi) {
+  constexpr int S = BF ? 2 * KT : 16 * KT;     // reduction steps
+  const int lane = threadIdx.x & 63, h = lane >> 5, rl = lane & 31;
+  const int64_t row = tile * 32 + rl;
+  const bool valid = row < M;
+  f32x16 acc[NTO];
+#pragma unroll
+  for (int t = 0; t < NTO; ++t)
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[t][v] = 0.f;
+  // rows past M read row M-1 (in bounds); their results are never stored
+  const int64_t rrow = valid ? row : M - 1;
+  if constexpr (!BF) {
+    // The B operand of reduction step s = 16 t + 4 u + w is input feature
+    // 32 t + 8 u + 4 h + w of this lane's row: one float4 per (t, u).  The 16 values
+    // of input tile t + 1 are loaded while tile t's 16 * NTO MFMAs run, so at most two
+    // tiles of operands are live beside the NTO accumulators.
+    const float* rp = reinterpret_cast<const float*>(in_) + rrow * in_stride + 4 * h;
+    const float4* lw = reinterpret_cast<const float4*>(lds_raw);
+    float4 cur[4], nxt[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) cur[u] = *reinterpret_cast<const float4*>(rp + 8 * u);
+#pragma unroll 1
+    for (int t = 0; t < KT; ++t) {
+      if (t + 1 < KT) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          nxt[u] = *reinterpret_cast<const float4*>(rp + 32 * (t + 1) + 8 * u);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int sq = 4 * t + u;
+#pragma unroll
+        for (int to = 0; to < NTO; ++to) {
+          if (!FULL && (to < to_lo || to >= to_hi)) continue;
+          const float4 a = lw[(to * (S / 4) + sq) * 64 + lane];
+          acc[to] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, cur[u].x, acc[to], 0, 0, 0);
+          acc[to] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.y, cur[u].y, acc[to], 0, 0, 0);
+          acc[to] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.z, cur[u].z, acc[to], 0, 0, 0);
+          acc[to] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.w, cur[u].w, acc[to], 0, 0, 0);
+        }
+      }
+      if (t + 1 < KT) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) cur[u] = nxt[u];
+      }
+    }
+  } else {
+    // BF16: step s takes the 8 features dfeat(s/2, 8 (s%2) + e, h), e = 0..7, which
+    // are 8 consecutive registers of the producing layer's output.  Input tile t
+    // (steps 2t, 2t+1) is loaded while tile t-1's MFMAs run.
+    const bf16x8* lw = reinterpret_cast<const bf16x8*>(lds_raw);
+    bf16x8 cur[2], nxt[2];
+    auto load_tile = [&](bf16x8* dst, int t) {
+      if constexpr (MODE == 0) {
+        // features 32 t + 8 u + 4 h + w for u = 0..3, w = 0..3, stored as bfloat16
+        const __bf16* in = reinterpret_cast<const __bf16*>(in_) + rrow * in_stride + 4 * h + 32 * t;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const uint2 lo = *reinterpret_cast<const uint2*>(in + 16 * q);
+          const uint2 hi = *reinterpret_cast<const uint2*>(in + 16 * q + 8);
+          uint4 v = make_uint4(lo.x, lo.y, hi.x, hi.y);
+          dst[q] = *reinterpret_cast<bf16x8*>(&v);
+        }
+      } else {
+        // backward: the incoming gradient rows are float32; rounded to bf16 here
+        const float* in = reinterpret_cast<const float*>(in_) + rrow * in_stride + 4 * h + 32 * t;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const float4 lo = *reinterpret_cast<const float4*>(in + 16 * q);
+          const float4 hi = *reinterpret_cast<const float4*>(in + 16 * q + 8);
+          dst[q][0] = (__bf16)lo.x;
+          dst[q][1] = (__bf16)lo.y;
+          dst[q][2] = (__bf16)lo.z;
+          dst[q][3] = (__bf16)lo.w;
+          dst[q][4] = (__bf16)hi.x;
+          dst[q][5] = (__bf16)hi.y;
+          dst[q][6] = (__bf16)hi.z;
+          dst[q][7] = (__bf16)hi.w;
+        }
+      }
+    };
+    load_tile(cur, 0);
+#pragma unroll 1
+    for (int t = 0; t < KT; ++t) {
+      if (t + 1 < KT) load_tile(nxt, t + 1);
+#pragma unroll
+      for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int to = 0; to < NTO; ++to)
+          if (FULL || (to >= to_lo && to < to_hi))
+            acc[to] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lw[(to * S + 2 * t + q) * 64 + lane],
+                                                               cur[q], acc[to], 0, 0, 0);
+      if (t + 1 < KT) {
+        cur[0] = nxt[0];
+        cur[1] = nxt[1];
+      }
+    }
+  }
+  // ---- epilogue ----
+  const int64_t obase = rrow * out_stride + 4 * h;
+  const int64_t mbase = rrow * mask_stride + 4 * h;
+#pragma unroll
+  for (int to = 0; to < NTO; ++to) {
+    if (!FULL && (to < to_lo || to >= to_hi)) continue;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      float v4[4];
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+        const int v = 4 * u + w;
+        float x = acc[to][v];
+        if (MODE == 0) {
+          x += lb[dfeat(to, v, h)];
+          x = x > 0.f ? x : 0.f;
+        }
+        v4[w] = x;
+      }
+      const int cofs = 32 * to + 8 * u;          // compile-time column offset
+      const int col = cofs + 4 * h;
+      if (MODE == 2 && valid) {
+        if constexpr (BF) {
+          const __bf16* mk = reinterpret_cast<const __bf16*>(mask_) + mbase + cofs;
+#pragma unroll
+          for (int w = 0; w < 4; ++w) v4[w] = (float)mk[w] > 0.f ? v4[w] : 0.f;
+        } else {
+          const float4 mk = *reinterpret_cast<const float4*>(reinterpret_cast<const float*>(mask_) +
+                                                            mbase + cofs);
+          v4[0] = mk.x > 0.f ? v4[0] : 0.f;
+          v4[1] = mk.y > 0.f ? v4[1] : 0.f;
+          v4[2] = mk.z > 0.f ? v4[2] : 0.f;
+          v4[3] = mk.w > 0.f ? v4[3] : 0.f;
+        }
+      }
+      if (valid) {
+        if (BF && MODE == 0) {
+          __bf16* o = reinterpret_cast<__bf16*>(out_) + obase + cofs;
+          __bf16 q[4] = {(__bf16)v4[0], (__bf16)v4[1], (__bf16)v4[2], (__bf16)v4[3]};
+          *reinterpret_cast<uint2*>(o) = *reinterpret_cast<uint2*>(q);
+        } else {
+          float* o = reinterpret_cast<float*>(out_) + obase + cofs;
+          *reinterpret_cast<float4*>(o) = make_float4(v4[0], v4[1], v4[2], v4[3]);
+        }
+      }
+    }
+  }
+}
+
 template <int KT, int NTO, int MODE, bool BF>
 __global__ void __launch_bounds__(kLayerThreads) k_mlp_layer(
     const void* __restrict__ in_, int in_stride, int64_t M, const void* __restrict__ wpk,
@@ -159,151 +313,27 @@ __global__ void __launch_bounds__(kLayerThreads) k_mlp_layer(
       for (int i = tid; i < NTO * 32; i += blockDim.x) lb[i] = bias[i];
   }
   __syncthreads();
-  const int lane = tid & 63, wave = tid >> 6, h = lane >> 5, rl = lane & 31;
+  // Stream-K split: the ntiles x NTO (32 x 32 output block) units are divided evenly
+  // over the grid's waves, so every SIMD gets the same MFMA work; a wave's first and last
+  // tiles may be partial (a subset of the output column tiles).
+  const int wave = tid >> 6;
   const int64_t ntiles = (M + 31) / 32;
-  const int nwaves = blockDim.x >> 6;
-  for (int64_t tile = (int64_t)blockIdx.x * nwaves + wave; tile < ntiles;
-       tile += (int64_t)gridDim.x * nwaves) {
-    const int64_t row = tile * 32 + rl;
-    const bool valid = row < M;
-    f32x16 acc[NTO];
-#pragma unroll
-    for (int t = 0; t < NTO; ++t)
-#pragma unroll
-      for (int v = 0; v < 16; ++v) acc[t][v] = 0.f;
-    // rows past M read row M-1 (in bounds); their results are never stored
-    const int64_t rrow = valid ? row : M - 1;
-    if constexpr (!BF) {
-      // The B operand of reduction step s = 16 t + 4 u + w is input feature
-      // 32 t + 8 u + 4 h + w of this lane's row: one float4 per (t, u).  The 16 values
-      // of input tile t + 1 are loaded while tile t's 16 * NTO MFMAs run, so at most two
-      // tiles of operands are live beside the NTO accumulators.
-      const float* rp = reinterpret_cast<const float*>(in_) + rrow * in_stride + 4 * h;
-      const float4* lw = reinterpret_cast<const float4*>(lds_raw);
-      float4 cur[4], nxt[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) cur[u] = *reinterpret_cast<const float4*>(rp + 8 * u);
-#pragma unroll 1
-      for (int t = 0; t < KT; ++t) {
-        if (t + 1 < KT) {
-#pragma unroll
-          for (int u = 0; u < 4; ++u)
-            nxt[u] = *reinterpret_cast<const float4*>(rp + 32 * (t + 1) + 8 * u);
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int sq = 4 * t + u;
-#pragma unroll
-          for (int to = 0; to < NTO; ++to) {
-            const float4 a = lw[(to * (S / 4) + sq) * 64 + lane];
-            acc[to] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, cur[u].x, acc[to], 0, 0, 0);
-            acc[to] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.y, cur[u].y, acc[to], 0, 0, 0);
-            acc[to] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.z, cur[u].z, acc[to], 0, 0, 0);
-            acc[to] = __builtin_amdgcn_mfma_f32_32x32x2f32(a.w, cur[u].w, acc[to], 0, 0, 0);
-          }
-        }
-        if (t + 1 < KT) {
-#pragma unroll
-          for (int u = 0; u < 4; ++u) cur[u] = nxt[u];
-        }
-      }
-    } else {
-      // BF16: step s takes the 8 features dfeat(s/2, 8 (s%2) + e, h), e = 0..7, which
-      // are 8 consecutive registers of the producing layer's output.  Input tile t
-      // (steps 2t, 2t+1) is loaded while tile t-1's MFMAs run.
-      const bf16x8* lw = reinterpret_cast<const bf16x8*>(lds_raw);
-      bf16x8 cur[2], nxt[2];
-      auto load_tile = [&](bf16x8* dst, int t) {
-        if constexpr (MODE == 0) {
-          // features 32 t + 8 u + 4 h + w for u = 0..3, w = 0..3, stored as bfloat16
-          const __bf16* in = reinterpret_cast<const __bf16*>(in_) + rrow * in_stride + 4 * h + 32 * t;
-#pragma unroll
-          for (int q = 0; q < 2; ++q) {
-            const uint2 lo = *reinterpret_cast<const uint2*>(in + 16 * q);
-            const uint2 hi = *reinterpret_cast<const uint2*>(in + 16 * q + 8);
-            uint4 v = make_uint4(lo.x, lo.y, hi.x, hi.y);
-            dst[q] = *reinterpret_cast<bf16x8*>(&v);
-          }
-        } else {
-          // backward: the incoming gradient rows are float32; rounded to bf16 here
-          const float* in = reinterpret_cast<const float*>(in_) + rrow * in_stride + 4 * h + 32 * t;
-#pragma unroll
-          for (int q = 0; q < 2; ++q) {
-            const float4 lo = *reinterpret_cast<const float4*>(in + 16 * q);
-            const float4 hi = *reinterpret_cast<const float4*>(in + 16 * q + 8);
-            dst[q][0] = (__bf16)lo.x;
-            dst[q][1] = (__bf16)lo.y;
-            dst[q][2] = (__bf16)lo.z;
-            dst[q][3] = (__bf16)lo.w;
-            dst[q][4] = (__bf16)hi.x;
-            dst[q][5] = (__bf16)hi.y;
-            dst[q][6] = (__bf16)hi.z;
-            dst[q][7] = (__bf16)hi.w;
-          }
-        }
-      };
-      load_tile(cur, 0);
-#pragma unroll 1
-      for (int t = 0; t < KT; ++t) {
-        if (t + 1 < KT) load_tile(nxt, t + 1);
-#pragma unroll
-        for (int q = 0; q < 2; ++q)
-#pragma unroll
-          for (int to = 0; to < NTO; ++to)
-            acc[to] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lw[(to * S + 2 * t + q) * 64 + lane],
-                                                               cur[q], acc[to], 0, 0, 0);
-        if (t + 1 < KT) {
-          cur[0] = nxt[0];
-          cur[1] = nxt[1];
-        }
-      }
-    }
-    // ---- epilogue ----
-    const int64_t obase = rrow * out_stride + 4 * h;
-    const int64_t mbase = rrow * mask_stride + 4 * h;
-#pragma unroll
-    for (int to = 0; to < NTO; ++to) {
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        float v4[4];
-#pragma unroll
-        for (int w = 0; w < 4; ++w) {
-          const int v = 4 * u + w;
-          float x = acc[to][v];
-          if (MODE == 0) {
-            x += lb[dfeat(to, v, h)];
-            x = x > 0.f ? x : 0.f;
-          }
-          v4[w] = x;
-        }
-        const int cofs = 32 * to + 8 * u;          // compile-time column offset
-        const int col = cofs + 4 * h;
-        if (MODE == 2 && valid) {
-          if constexpr (BF) {
-            const __bf16* mk = reinterpret_cast<const __bf16*>(mask_) + mbase + cofs;
-#pragma unroll
-            for (int w = 0; w < 4; ++w) v4[w] = (float)mk[w] > 0.f ? v4[w] : 0.f;
-          } else {
-            const float4 mk = *reinterpret_cast<const float4*>(reinterpret_cast<const float*>(mask_) +
-                                                              mbase + cofs);
-            v4[0] = mk.x > 0.f ? v4[0] : 0.f;
-            v4[1] = mk.y > 0.f ? v4[1] : 0.f;
-            v4[2] = mk.z > 0.f ? v4[2] : 0.f;
-            v4[3] = mk.w > 0.f ? v4[3] : 0.f;
-          }
-        }
-        if (valid) {
-          if (BF && MODE == 0) {
-            __bf16* o = reinterpret_cast<__bf16*>(out_) + obase + cofs;
-            __bf16 q[4] = {(__bf16)v4[0], (__bf16)v4[1], (__bf16)v4[2], (__bf16)v4[3]};
-            *reinterpret_cast<uint2*>(o) = *reinterpret_cast<uint2*>(q);
-          } else {
-            float* o = reinterpret_cast<float*>(out_) + obase + cofs;
-            *reinterpret_cast<float4*>(o) = make_float4(v4[0], v4[1], v4[2], v4[3]);
-          }
-        }
-      }
-    }
+  const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
+  const int64_t gw = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
+  const int64_t total = ntiles * NTO;
+  int64_t u = total * gw / nw;
+  const int64_t u1 = total * (gw + 1) / nw;
+  while (u < u1) {
+    const int64_t tile = u / NTO;
+    const int lo = (int)(u - tile * NTO);
+    const int hi = (u1 - u) < (int64_t)(NTO - lo) ? lo + (int)(u1 - u) : NTO;
+    if (lo == 0 && hi == NTO)
+      layer_tile<KT, NTO, MODE, BF, true>(in_, in_stride, M, lds_raw, lb, mask_, mask_stride, out_,
+                                          out_stride, tile, 0, NTO);
+    else
+      layer_tile<KT, NTO, MODE, BF, false>(in_, in_stride, M, lds_raw, lb, mask_, mask_stride, out_,
+                                           out_stride, tile, lo, hi);
+    u += hi - lo;
   }
 }
 
@@ -597,8 +627,10 @@ template <int KT, int NTO, int MODE, bool BF>
 static void launch_layer(const void* in, int in_stride, int64_t M, const void* wpk,
                          const float* bias, const void* mask, int mask_stride, void* out,
                          int out_stride, hipStream_t st) {
+  // about two 32 x 32 output blocks per wave at least (the stream-K split evens out the
+  // rest), at most one block per CU (the weights fill its LDS)
   const int64_t tiles = (M + 31) / 32;
-  int64_t blocks = (tiles + 7) / 8;
+  int64_t blocks = (tiles * NTO + 15) / 16;
   const int64_t cap = num_cus();
   if (blocks > cap) blocks = cap;
   if (blocks < 1) blocks = 1;
