@@ -363,44 +363,52 @@ int vx_fused_fine_fwd(const float* sdf, const int64_t* sdf_geom, const float* bo
  * off_incl[r]-cnt[r] .. off_incl[r]-1 and builds the rgbnet input rows
  * X [M, xstride] = [k0 | normal | view PE | 0] (lib/voxurf_fine.py:678-716; xstride a
  * multiple of 4, at most 64); P4 [M] = (x, y, z, weight), G4 [M] = (gradient, sdf),
- * rid [M] int32 ray ids.  k0_pages (may be NULL): set the page flag (vx_adam_step) of
- * every k0_grad page the feature scatter of these rows can write, so that the k0 Adam
- * pass over the other pages can start before the backward. */
+ * rid [M] int32 ray ids; rows at or past max_rows (the buffers' capacity) are not
+ * written, so the call may be issued before M is known on the host and repeated with
+ * larger buffers when M exceeds the capacity.  k0_pages (may be NULL): set the page flag
+ * (vx_adam_step) of every k0_grad page the feature scatter of these rows can write, so
+ * that the k0 Adam pass over the other pages can start before the backward. */
 int vx_fused_fine_gather(const float* k0, const int64_t* k0_geom, const int64_t* sdf_geom,
                          const float* box, const float* params, const float* viewdirs,
                          int64_t n_rays, int cap, int n_freq, const void* s4, const void* g4,
                          const int* stats, const int64_t* cnt, const int64_t* off_incl,
-                         int xstride, float* X, void* P4, void* G4, int* rid,
+                         int xstride, int64_t max_rows, float* X, void* P4, void* G4, int* rid,
                          uint32_t* k0_pages, void* stream);
 /* segment_coo compositing (lib/voxurf_fine.py:753-763), MSE loss per ray (run.py:604)
  * and the colour-side adjoints: dlogit [M,3], gw [M] (d loss / d weight), glast [N],
  * S [N] (sum of gw*w per ray), loss_r [N] (sum of squared errors).  With target NULL
- * only rgb [N,3] is written (rendering; the adjoint pointers may be NULL). */
+ * only rgb [N,3] is written (rendering; the adjoint pointers may be NULL).  Rays whose
+ * rows reach past max_rows (the row buffers' capacity) are left out. */
 int vx_fused_fine_composite(int64_t n_rays, const int64_t* cnt, const int64_t* off_incl,
                             const float* logits, int lstride, const void* P4, const float* last,
                             const float* target, float bg, float* rgb, float* dlogit, float* gw,
-                            float* glast, float* S, float* loss_r, void* stream);
+                            float* glast, float* S, float* loss_r, int64_t max_rows,
+                            void* stream);
 /* Adjoint of the feature rows: k0_grad += trilinear scatter of dX[:, :C]; dgf [M,3] =
  * d loss / d gradient through the normalised normal dX[:, C:C+3].  k0_pages (may be
- * NULL): the page flags of k0_grad for vx_adam_step, set for every page written. */
+ * NULL): the page flags of k0_grad for vx_adam_step, set for every page written.  m_dev
+ * (may be NULL): device row count, min(*m_dev, m) rows are processed. */
 int vx_fused_fine_feat_bwd(const int64_t* k0_geom, const int64_t* sdf_geom, const float* box,
                            const float* params, int64_t m, const float* dX, int dxs,
                            const void* P4, const void* G4, float* k0_grad, float* dgf,
-                           uint32_t* k0_pages, void* stream);
+                           uint32_t* k0_pages, const int64_t* m_dev, void* stream);
 /* Backward of the SDF side over each ray's stored walk range: sdf_grad += adjoint of
- * (7-point SDF lookup -> NeuS alpha -> transmittance walk). */
+ * (7-point SDF lookup -> NeuS alpha -> transmittance walk).  Rays whose rows reach past
+ * max_rows are left out, as in the composite. */
 int vx_fused_fine_bwd(const float* sdf, const int64_t* sdf_geom, const float* box,
                       const float* params, const float* viewdirs, int64_t n_rays, int cap,
                       const void* s4, const void* g4, const int* stats, const int64_t* cnt,
                       const int64_t* off_incl, const float* last, const float* gw,
                       const float* glast, const float* S, const float* dgf, float* sdf_grad,
-                      void* stream);
+                      int64_t max_rows, void* stream);
 /* ---- rgbnet on the matrix cores ---------------------------------------------------
  * The colour MLP of lib/voxurf_fine.py:132-149 (nn.Linear/ReLU layers of width 192,
  * which the reference runs as cuBLAS GEMMs plus elementwise kernels).  bf16 = 0: FP32
  * MFMAs; bf16 = 1: BF16 MFMAs with FP32 accumulation and BF16 activations.
  * Weights are packed per call into the MFMA operand order; activation rows are
- * [M, 192] (element type per bf16), inputs [M, 32*kt] zero padded. */
+ * [M, 192] (element type per bf16), inputs [M, 32*kt] zero padded.  M_dev (may be NULL):
+ * a device int64 row count; when given the calls process min(*M_dev, M) rows, M being
+ * the buffers' capacity, so they can be queued before the count is known on the host. */
 
 /* Pack W [n_out, k_in] (row-major, float32) as the A operand of a layer with kt input
  * tiles of 32 (reduction) and nto output tiles; transpose = 1 packs W^T (backward). */
@@ -419,11 +427,12 @@ int vx_mlp_pack_net(int bf16, int n_hidden, const int64_t* w_ptrs, int k_in0, vo
 /* One hidden layer: out = relu(W in + b); with last = 1 also logits [M,3] = W3 out + b3. */
 int vx_mlp_layer_fwd(int bf16, int kt, int last, const void* in, int in_stride, int64_t M,
                      const void* wpk, const float* bias, void* out, int out_stride,
-                     const float* w3, const float* b3, float* logits, void* stream);
+                     const float* w3, const float* b3, float* logits, const int64_t* M_dev,
+                     void* stream);
 /* Data gradient through a layer: dx = (W^T dy) * [mask > 0] (mask NULL: no ReLU). */
 int vx_mlp_layer_bwd(int bf16, int nto, const float* dy, int dy_stride, int64_t M,
                      const void* wpk_t, const void* mask, int mask_stride, float* dx,
-                     int dx_stride, void* stream);
+                     int dx_stride, const int64_t* M_dev, void* stream);
 /* Floats the `part` / `partb` workspaces of vx_mlp_wgrad and vx_mlp_last_bwd need for
  * M rows (row-block partial sums plus the reduction's group sums). */
 int64_t vx_mlp_wgrad_part_elems(int64_t M);
@@ -431,12 +440,13 @@ int64_t vx_mlp_wgrad_partb_elems(int64_t M);
 /* dW [192, k_real] = dY^T X, db [192] = column sums of dY (split over row blocks, then
  * a deterministic two-pass reduction). */
 int vx_mlp_wgrad(int bf16, int kt, int k_real, const float* dY, int dys, const void* X, int xs,
-                 int64_t M, float* part, float* partb, float* dW, float* db, void* stream);
+                 int64_t M, float* part, float* partb, float* dW, float* db, const int64_t* M_dev,
+                 void* stream);
 /* Backward through the 3-output layer in one pass over H: dH = (dlogit W3) * [H > 0],
  * dW3 [3, 192] = dlogit^T H, db3 [3] = column sums of dlogit. */
 int vx_mlp_last_bwd(int bf16, const float* dlogit, const void* H, int hs, int64_t M,
                     const float* w3, float* dH, int dhs, float* part, float* partb, float* dW3,
-                    float* db3, void* stream);
+                    float* db3, const int64_t* M_dev, void* stream);
 /* y = bfloat16(x), n elements. */
 int vx_f32_to_bf16(const float* x, int64_t n, void* y, void* stream);
 

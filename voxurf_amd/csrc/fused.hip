@@ -211,8 +211,8 @@ __global__ void __launch_bounds__(256) k_fused_gather(
     FusedGeom G, const float* __restrict__ viewdirs, int64_t n_rays, int cap, int n_freq,
     const float4* __restrict__ s4, const float4* __restrict__ g4, const int* __restrict__ stats,
     const int64_t* __restrict__ cnt, const int64_t* __restrict__ off_incl, int xstride,
-    float* __restrict__ X, float4* __restrict__ P4, float4* __restrict__ G4, int* __restrict__ rid,
-    uint32_t* __restrict__ pages) {
+    int64_t max_rows, float* __restrict__ X, float4* __restrict__ P4, float4* __restrict__ G4,
+    int* __restrict__ rid, uint32_t* __restrict__ pages) {
   const int lane = threadIdx.x & 63, grp = lane >> 4, cl = lane & 15;
   const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
   const int C = G.gk.C;
@@ -226,6 +226,7 @@ __global__ void __launch_bounds__(256) k_fused_gather(
     if (c == 0) continue;
     const int c1 = stats[3 * r + 1];
     const int64_t o = off_incl[r] - c;
+    if (o >= max_rows && !pages) continue;  // past the output capacity (rows are not written)
     const float* vd = viewdirs + 3 * r;
     // view PE values of this lane's four columns (zero outside the PE block)
     float pev[4];
@@ -263,7 +264,7 @@ __global__ void __launch_bounds__(256) k_fused_gather(
       // the recomputed weights are the forward's)
       const int64_t room = c - ks;
       const int rank = __popcll(sm & lanemask_lt(lane));
-      if (surv && rank < room) {
+      if (surv && rank < room && o + ks + rank < max_rows) {
         const int64_t j = o + ks + rank;
         P4[j] = make_float4(a.x, a.y, a.z, w);
         G4[j] = b;
@@ -282,7 +283,7 @@ __global__ void __launch_bounds__(256) k_fused_gather(
                  world_to_index(qy, G.mn[1], G.mx[1] - G.mn[1], G.gk.Y),
                  world_to_index(qz, G.mn[2], G.mx[2] - G.mn[2], G.gk.Z));
           if (pages && cl < 8) mark_corner_pages(pages, G.gk, t, cl);
-          if (4 * cl < xstride) {
+          if (4 * cl < xstride && o + ks + kk < max_rows) {
             const float nrm = sqrtf(gx * gx + gy * gy + gz * gz) + 1e-5f;
             float v[4];
             if (vec4 && 4 * cl + 3 < C) {
@@ -328,10 +329,13 @@ __global__ void k_fused_composite(int64_t n_rays, const int64_t* __restrict__ cn
                                   const float* __restrict__ target, float bg, float inv_count,
                                   float* __restrict__ rgb_out, float* __restrict__ dlogit,
                                   float* __restrict__ gw, float* __restrict__ glast,
-                                  float* __restrict__ S, float* __restrict__ loss_r) {
+                                  float* __restrict__ S, float* __restrict__ loss_r,
+                                  int64_t max_rows) {
   for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n_rays;
        r += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t c = cnt[r], o = off_incl[r] - c;
+    // a ray whose rows run past the buffers' capacity is left out (the caller repeats
+    // the step with larger buffers)
+    const int64_t c = off_incl[r] <= max_rows ? cnt[r] : 0, o = off_incl[r] - cnt[r];
     float acc[3] = {0.f, 0.f, 0.f};
     for (int64_t j = o; j < o + c; ++j) {
       const float w = P4[j].w;
@@ -374,9 +378,10 @@ __global__ void k_fused_composite(int64_t n_rays, const int64_t* __restrict__ cn
 // dX[:, C:C+3] (normal) -> d grad through n = g / (|g| + 1e-5); result stored in dgf.
 
 __global__ void __launch_bounds__(256) k_fused_feat_bwd(
-    FusedGeom G, int64_t m, const float* __restrict__ dX, int dxs,
-    const float4* __restrict__ P4, const float4* __restrict__ G4, float* __restrict__ k0_grad,
-    float* __restrict__ dgf, uint32_t* __restrict__ pages) {
+    FusedGeom G, int64_t m, const int64_t* __restrict__ m_dev, const float* __restrict__ dX,
+    int dxs, const float4* __restrict__ P4, const float4* __restrict__ G4,
+    float* __restrict__ k0_grad, float* __restrict__ dgf, uint32_t* __restrict__ pages) {
+  if (m_dev) m = *m_dev < m ? *m_dev : m;  // the device row count, within the capacity
   const int C = G.gk.C;
   const int lane = threadIdx.x & 63, grp = lane >> 4, cl = lane & 15;
   const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
@@ -424,7 +429,8 @@ __global__ void __launch_bounds__(256) k_fused_bwd(
     const float4* __restrict__ s4, const float4* __restrict__ g4, const int* __restrict__ stats,
     const int64_t* __restrict__ cnt, const int64_t* __restrict__ off_incl,
     const float* __restrict__ last, const float* __restrict__ gw, const float* __restrict__ glast,
-    const float* __restrict__ S, const float* __restrict__ dgf, float* __restrict__ sdf_grad) {
+    const float* __restrict__ S, const float* __restrict__ dgf, float* __restrict__ sdf_grad,
+    int64_t max_rows) {
   constexpr int SEG = 16;
   const int lane = threadIdx.x & 63, grp = lane / SEG, sl = lane % SEG;
   const uint64_t seg_lt = sl ? ((1ull << sl) - 1ull) : 0ull;
@@ -432,7 +438,8 @@ __global__ void __launch_bounds__(256) k_fused_bwd(
   for (int64_t r0 = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * (64 / SEG);
        r0 < n_rays; r0 += groups) {
     const int64_t r = r0 + grp;
-    const bool ray_ok = r < n_rays;
+    // (a ray whose rows run past the buffers' capacity is left out, as in the composite)
+    const bool ray_ok = r < n_rays && off_incl[r] <= max_rows;
     const float* vd = viewdirs + 3 * (ray_ok ? r : 0);
     const int c1 = ray_ok ? stats[3 * r + 1] : 0;
     const int64_t c_r = ray_ok ? cnt[r] : 0;
@@ -538,7 +545,7 @@ int vx_fused_fine_gather(const float* k0, const int64_t* k0_geom, const int64_t*
                          const float* box, const float* params, const float* viewdirs,
                          int64_t n_rays, int cap, int n_freq, const void* s4, const void* g4,
                          const int* stats, const int64_t* cnt, const int64_t* off_incl,
-                         int xstride, float* X, void* P4, void* G4, int* rid,
+                         int xstride, int64_t max_rows, float* X, void* P4, void* G4, int* rid,
                          uint32_t* k0_pages, void* stream) {
   FusedGeom G;
   if (int st = fused_geom(G, nullptr, sdf_geom, k0, k0_geom, box, params)) return st;
@@ -548,32 +555,33 @@ int vx_fused_fine_gather(const float* k0, const int64_t* k0_geom, const int64_t*
   if (n_rays == 0) return VX_OK;
   k_fused_gather<<<grid_for(n_rays, 4, 1 << 16), 256, 0, (hipStream_t)stream>>>(
       G, viewdirs, n_rays, cap, n_freq, (const float4*)s4, (const float4*)g4, stats, cnt,
-      off_incl, xstride, X, (float4*)P4, (float4*)G4, rid, k0_pages);
+      off_incl, xstride, max_rows, X, (float4*)P4, (float4*)G4, rid, k0_pages);
   return launch_status("fused_fine_gather");
 }
 
 int vx_fused_fine_composite(int64_t n_rays, const int64_t* cnt, const int64_t* off_incl,
                             const float* logits, int lstride, const void* P4, const float* last,
                             const float* target, float bg, float* rgb, float* dlogit, float* gw,
-                            float* glast, float* S, float* loss_r, void* stream) {
+                            float* glast, float* S, float* loss_r, int64_t max_rows,
+                            void* stream) {
   VX_CHECK_ARG(n_rays >= 0, "fused_fine_composite: n_rays < 0");
   if (n_rays == 0) return VX_OK;
   const float inv_count = 1.f / (3.f * (float)n_rays);
   k_fused_composite<<<grid_for(n_rays, 64), 64, 0, (hipStream_t)stream>>>(
       n_rays, cnt, off_incl, logits, lstride, (const float4*)P4, last, target, bg, inv_count, rgb,
-      dlogit, gw, glast, S, loss_r);
+      dlogit, gw, glast, S, loss_r, max_rows);
   return launch_status("fused_fine_composite");
 }
 
 int vx_fused_fine_feat_bwd(const int64_t* k0_geom, const int64_t* sdf_geom, const float* box,
                            const float* params, int64_t m, const float* dX, int dxs,
                            const void* P4, const void* G4, float* k0_grad, float* dgf,
-                           uint32_t* k0_pages, void* stream) {
+                           uint32_t* k0_pages, const int64_t* m_dev, void* stream) {
   FusedGeom G;
   if (int st = fused_geom(G, nullptr, sdf_geom, nullptr, k0_geom, box, params)) return st;
   if (m == 0) return VX_OK;
   k_fused_feat_bwd<<<grid_for(m, 256, 1 << 16), 256, 0, (hipStream_t)stream>>>(
-      G, m, dX, dxs, (const float4*)P4, (const float4*)G4, k0_grad, dgf, k0_pages);
+      G, m, m_dev, dX, dxs, (const float4*)P4, (const float4*)G4, k0_grad, dgf, k0_pages);
   return launch_status("fused_fine_feat_bwd");
 }
 
@@ -582,14 +590,14 @@ int vx_fused_fine_bwd(const float* sdf, const int64_t* sdf_geom, const float* bo
                       const void* s4, const void* g4, const int* stats, const int64_t* cnt,
                       const int64_t* off_incl, const float* last, const float* gw,
                       const float* glast, const float* S, const float* dgf, float* sdf_grad,
-                      void* stream) {
+                      int64_t max_rows, void* stream) {
   FusedGeom G;
   if (int st = fused_geom(G, sdf, sdf_geom, nullptr, nullptr, box, params)) return st;
   VX_CHECK_ARG(cap >= 1, "fused_fine_bwd: cap < 1");
   if (n_rays == 0) return VX_OK;
   k_fused_bwd<<<grid_for(n_rays, 16, 1 << 16), 256, 0, (hipStream_t)stream>>>(
       G, viewdirs, n_rays, cap, (const float4*)s4, (const float4*)g4, stats, cnt, off_incl, last,
-      gw, glast, S, dgf, sdf_grad);
+      gw, glast, S, dgf, sdf_grad, max_rows);
   return launch_status("fused_fine_bwd");
 }
 

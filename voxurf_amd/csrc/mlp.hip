@@ -117,6 +117,14 @@ __global__ void k_pack_net(PackNet net, void* __restrict__ out) {
   }
 }
 
+// Rows to process: M, or the device count *Md (clamped to M, the buffers' capacity) when
+// given, so that a launch can be queued before the count is known on the host.
+__device__ __forceinline__ int64_t eff_rows(int64_t M, const int64_t* Md) {
+  if (!Md) return M;
+  const int64_t m = *Md;
+  return m < M ? (m > 0 ? m : 0) : M;
+}
+
 // ---- one layer over all rows -------------------------------------------------------
 // MODE 0: out = relu(W x + b)            (hidden layer, forward)
 // MODE 2: out = (W^T dy) * [mask > 0]    (backward data through a ReLU layer)
@@ -283,9 +291,9 @@ __device__ __forceinline__ void layer_tile(const void* __restrict__ in_, int in_
 
 template <int KT, int NTO, int MODE, bool BF>
 __global__ void __launch_bounds__(kLayerThreads) k_mlp_layer(
-    const void* __restrict__ in_, int in_stride, int64_t M, const void* __restrict__ wpk,
-    const float* __restrict__ bias, const void* __restrict__ mask_, int mask_stride,
-    void* __restrict__ out_, int out_stride) {
+    const void* __restrict__ in_, int in_stride, int64_t M, const int64_t* __restrict__ Md,
+    const void* __restrict__ wpk, const float* __restrict__ bias, const void* __restrict__ mask_,
+    int mask_stride, void* __restrict__ out_, int out_stride) {
   constexpr int S = BF ? 2 * KT : 16 * KT;     // reduction steps
   constexpr int WELEMS = NTO * S * 64 * (BF ? 8 : 1);
   __shared__ __attribute__((aligned(16))) unsigned char lds_raw[WELEMS * (BF ? 2 : 4)];
@@ -317,6 +325,7 @@ __global__ void __launch_bounds__(kLayerThreads) k_mlp_layer(
   // over the grid's waves, so every SIMD gets the same MFMA work; a wave's first and last
   // tiles may be partial (a subset of the output column tiles).
   const int wave = tid >> 6;
+  M = eff_rows(M, Md);
   const int64_t ntiles = (M + 31) / 32;
   const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
   const int64_t gw = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave;
@@ -341,9 +350,11 @@ __global__ void __launch_bounds__(kLayerThreads) k_mlp_layer(
 // per row: lanes 0..47 take four features each, then a wave sum.
 template <bool BF>
 __global__ void __launch_bounds__(256) k_mlp_logits(const void* __restrict__ H_, int hs,
-                                                    int64_t M, const float* __restrict__ w3,
+                                                    int64_t M, const int64_t* __restrict__ Md,
+                                                    const float* __restrict__ w3,
                                                     const float* __restrict__ b3,
                                                     float* __restrict__ logits) {
+  M = eff_rows(M, Md);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const bool act = lane < kWidth / 4;
   const int f = act ? 4 * lane : 0;
@@ -384,13 +395,15 @@ __global__ void __launch_bounds__(256) k_mlp_logits(const void* __restrict__ H_,
 template <bool BF>
 __global__ void __launch_bounds__(192) k_mlp_last_bwd(const float* __restrict__ dlogit,
                                                       const void* __restrict__ H_, int hs,
-                                                      int64_t M, int64_t rows_per_block,
+                                                      int64_t M, const int64_t* __restrict__ Md,
                                                       const float* __restrict__ w3,
                                                       float* __restrict__ dH, int dhs,
                                                       float* __restrict__ part3,
                                                       float* __restrict__ partb3) {
   const int f = threadIdx.x;
   const float w0 = w3[f], w1 = w3[kWidth + f], w2 = w3[2 * kWidth + f];
+  M = eff_rows(M, Md);
+  const int64_t rows_per_block = (M + gridDim.x - 1) / gridDim.x;
   const int64_t r_begin = blockIdx.x * rows_per_block;
   const int64_t r_end = (r_begin + rows_per_block < M) ? r_begin + rows_per_block : M;
   float a0 = 0.f, a1 = 0.f, a2 = 0.f, c0 = 0.f, c1 = 0.f, c2 = 0.f;
@@ -440,7 +453,8 @@ constexpr int kWgradChunk = 32;
 template <int KT, bool BF>
 __global__ void __launch_bounds__(256) k_mlp_wgrad(const float* __restrict__ dY, int dys,
                                                    const void* __restrict__ X_, int xs, int64_t M,
-                                                   int64_t rows_per_block, float* __restrict__ part,
+                                                   const int64_t* __restrict__ Md,
+                                                   float* __restrict__ part,
                                                    float* __restrict__ partb) {
   using Sh = WgradShape<KT>;
   constexpr int K = 32 * KT, TNW = Sh::TNW, TKW = Sh::TKW, NWN = Sh::NWN;
@@ -453,6 +467,9 @@ __global__ void __launch_bounds__(256) k_mlp_wgrad(const float* __restrict__ dY,
   __shared__ __attribute__((aligned(16))) float sX[2][CH][K];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, h = lane >> 5, rl = lane & 31;
   const int tn0 = TNW * (wave % NWN), tk0 = TKW * (wave / NWN);
+  M = eff_rows(M, Md);
+  // contiguous row ranges of whole chunks, one per block
+  const int64_t rows_per_block = ((M + gridDim.x - 1) / gridDim.x + CH - 1) / CH * CH;
   const int64_t r_begin = blockIdx.x * rows_per_block;
   const int64_t r_end = (r_begin + rows_per_block < M) ? r_begin + rows_per_block : M;
 
@@ -624,9 +641,9 @@ static int num_cus() {
 }
 
 template <int KT, int NTO, int MODE, bool BF>
-static void launch_layer(const void* in, int in_stride, int64_t M, const void* wpk,
-                         const float* bias, const void* mask, int mask_stride, void* out,
-                         int out_stride, hipStream_t st) {
+static void launch_layer(const void* in, int in_stride, int64_t M, const int64_t* Md,
+                         const void* wpk, const float* bias, const void* mask, int mask_stride,
+                         void* out, int out_stride, hipStream_t st) {
   // about two 32 x 32 output blocks per wave at least (the stream-K split evens out the
   // rest), at most one block per CU (the weights fill its LDS)
   const int64_t tiles = (M + 31) / 32;
@@ -635,7 +652,7 @@ static void launch_layer(const void* in, int in_stride, int64_t M, const void* w
   if (blocks > cap) blocks = cap;
   if (blocks < 1) blocks = 1;
   k_mlp_layer<KT, NTO, MODE, BF><<<(unsigned)blocks, kLayerThreads, 0, st>>>(
-      in, in_stride, M, wpk, bias, mask, mask_stride, out, out_stride);
+      in, in_stride, M, Md, wpk, bias, mask, mask_stride, out, out_stride);
 }
 
 // Row blocks of the weight-gradient kernel: one per CU (each holds two row chunks in
@@ -730,13 +747,14 @@ int vx_mlp_pack_net(int bf16, int n_hidden, const int64_t* w_ptrs, int k_in0, vo
 
 int vx_mlp_layer_fwd(int bf16, int kt, int last, const void* in, int in_stride, int64_t M,
                      const void* wpk, const float* bias, void* out, int out_stride,
-                     const float* w3, const float* b3, float* logits, void* stream) {
+                     const float* w3, const float* b3, float* logits, const int64_t* M_dev,
+                     void* stream) {
   VX_CHECK_ARG(M >= 0, "mlp_layer_fwd: M < 0");
   VX_CHECK_ARG(kt == 2 || kt == 3 || kt == 6, "mlp_layer_fwd: input width must be 64, 96 or 192");
   if (M == 0) return VX_OK;
   VX_CHECK_ARG(!last || (w3 && b3 && logits), "mlp_layer_fwd: last layer needs w3, b3, logits");
   hipStream_t st = (hipStream_t)stream;
-#define L(KT, BF) launch_layer<KT, 6, 0, BF>(in, in_stride, M, wpk, bias, nullptr, 0, out, out_stride, st)
+#define L(KT, BF) launch_layer<KT, 6, 0, BF>(in, in_stride, M, M_dev, wpk, bias, nullptr, 0, out, out_stride, st)
   if (!bf16) {
     if (kt == 2) L(2, false);
     else if (kt == 3) L(3, false);
@@ -749,20 +767,20 @@ int vx_mlp_layer_fwd(int bf16, int kt, int last, const void* in, int in_stride, 
 #undef L
   if (last) {
     const unsigned g = grid_for(M, 4, 8 * (int64_t)num_cus());
-    if (bf16) k_mlp_logits<true><<<g, 256, 0, st>>>(out, out_stride, M, w3, b3, logits);
-    else k_mlp_logits<false><<<g, 256, 0, st>>>(out, out_stride, M, w3, b3, logits);
+    if (bf16) k_mlp_logits<true><<<g, 256, 0, st>>>(out, out_stride, M, M_dev, w3, b3, logits);
+    else k_mlp_logits<false><<<g, 256, 0, st>>>(out, out_stride, M, M_dev, w3, b3, logits);
   }
   return launch_status("mlp_layer_fwd");
 }
 
 int vx_mlp_layer_bwd(int bf16, int nto, const float* dy, int dy_stride, int64_t M,
                      const void* wpk_t, const void* mask, int mask_stride, float* dx,
-                     int dx_stride, void* stream) {
+                     int dx_stride, const int64_t* M_dev, void* stream) {
   VX_CHECK_ARG(M >= 0, "mlp_layer_bwd: M < 0");
   VX_CHECK_ARG(nto == 2 || nto == 3 || nto == 6, "mlp_layer_bwd: output width must be 64, 96 or 192");
   if (M == 0) return VX_OK;
   hipStream_t st = (hipStream_t)stream;
-#define L(NTO, MODE, BF) launch_layer<6, NTO, MODE, BF>(dy, dy_stride, M, wpk_t, nullptr, mask, mask_stride, dx, dx_stride, st)
+#define L(NTO, MODE, BF) launch_layer<6, NTO, MODE, BF>(dy, dy_stride, M, M_dev, wpk_t, nullptr, mask, mask_stride, dx, dx_stride, st)
   if (!bf16) {
     if (nto == 6) { if (mask) L(6, 2, false); else L(6, 3, false); }
     else if (nto == 3) L(3, 3, false);
@@ -794,15 +812,15 @@ int64_t vx_mlp_wgrad_partb_elems(int64_t M) {
 }
 
 int vx_mlp_wgrad(int bf16, int kt, int k_real, const float* dY, int dys, const void* X, int xs,
-                 int64_t M, float* part, float* partb, float* dW, float* db, void* stream) {
+                 int64_t M, float* part, float* partb, float* dW, float* db, const int64_t* M_dev,
+                 void* stream) {
   VX_CHECK_ARG(kt == 2 || kt == 3 || kt == 6, "mlp_wgrad: input width must be 64, 96 or 192");
   VX_CHECK_ARG(k_real >= 1 && k_real <= 32 * kt, "mlp_wgrad: k_real out of range");
   VX_CHECK_ARG(M >= 0, "mlp_wgrad: M < 0");
   hipStream_t st = (hipStream_t)stream;
   const int nblk = wgrad_blocks(M, kt);
-  const int64_t rpb = cdiv(cdiv(M, nblk), kWgradChunk) * kWgradChunk;
   const int K = 32 * kt;
-#define L(KT, BF) k_mlp_wgrad<KT, BF><<<nblk, 64 * WgradShape<KT>::WAVES, 0, st>>>(dY, dys, X, xs, M, rpb, part, partb)
+#define L(KT, BF) k_mlp_wgrad<KT, BF><<<nblk, 64 * WgradShape<KT>::WAVES, 0, st>>>(dY, dys, X, xs, M, M_dev, part, partb)
   if (kt == 2) { if (bf16) L(2, true); else L(2, false); }
   else if (kt == 3) { if (bf16) L(3, true); else L(3, false); }
   else { if (bf16) L(6, true); else L(6, false); }
@@ -815,13 +833,12 @@ int vx_mlp_wgrad(int bf16, int kt, int k_real, const float* dY, int dys, const v
 
 int vx_mlp_last_bwd(int bf16, const float* dlogit, const void* H, int hs, int64_t M,
                     const float* w3, float* dH, int dhs, float* part, float* partb, float* dW3,
-                    float* db3, void* stream) {
+                    float* db3, const int64_t* M_dev, void* stream) {
   VX_CHECK_ARG(M >= 0, "mlp_last_bwd: M < 0");
   hipStream_t st = (hipStream_t)stream;
   const int nblk = last_blocks(M);
-  const int64_t rpb = cdiv(M, nblk);
-  if (bf16) k_mlp_last_bwd<true><<<nblk, kWidth, 0, st>>>(dlogit, H, hs, M, rpb, w3, dH, dhs, part, partb);
-  else k_mlp_last_bwd<false><<<nblk, kWidth, 0, st>>>(dlogit, H, hs, M, rpb, w3, dH, dhs, part, partb);
+  if (bf16) k_mlp_last_bwd<true><<<nblk, kWidth, 0, st>>>(dlogit, H, hs, M, M_dev, w3, dH, dhs, part, partb);
+  else k_mlp_last_bwd<false><<<nblk, kWidth, 0, st>>>(dlogit, H, hs, M, M_dev, w3, dH, dhs, part, partb);
   const RedJob jw{part, part + (int64_t)nblk * 3 * kWidth, dW3, 3 * kWidth, 3, kWidth, kWidth};
   const RedJob jb{partb, partb + (int64_t)nblk * 3, db3, 3, 1, 3, 3};
   reduce_two(jw, jb, nblk, st);

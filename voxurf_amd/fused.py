@@ -77,6 +77,11 @@ class FusedFineTrainer:
                        for _ in range(2)]
         self._parity = 0
         self.n_pages = n_pages
+        self._rows = 0          # capacity of the survivor-row buffers (grown on demand)
+        self._grow_rows(0)
+        # the survivor count of the last forward, copied to pinned host memory
+        self._m_host = torch.zeros(1, dtype=torch.int64, pin_memory=True)
+        self._m_ready = torch.cuda.Event()
         # One word per k0 page, nonzero once any step has flagged the page: the split
         # Adam skips unflagged pages that were never touched, whose moments are still
         # zero (the update is then the identity, as in utils.Adam).
@@ -203,7 +208,7 @@ class FusedFineTrainer:
         """Steps 1-3: survivors and rgbnet input rows; returns M (survivor count)."""
         return self._forward(rays_o, rays_d, viewdirs, None, None)
 
-    def _forward(self, rays_o, rays_d, viewdirs, pages, k0_step):
+    def _forward(self, rays_o, rays_d, viewdirs, pages, k0_step, sync=True):
         n = rays_o.shape[0]
         self._ensure(n)
         st = stream_of(self.sdf)
@@ -226,22 +231,60 @@ class FusedFineTrainer:
                 self._adam("k0", self._k0, self.k0_grad, pages=pages, phase=1, step=k0_step,
                            timed=True)
         call("vx_inclusive_scan_i64", ptr(self.cnt), ptr(self.off), n, ptr(self.ws), st)
-        m = int(self.off[-1].item()) if n > 0 else 0
-        self.m = m
-        self.X = torch.empty((max(m, 1), XSTRIDE), dtype=torch.float32, device=self.dev)
-        self.P4 = torch.empty((max(m, 1), 4), dtype=torch.float32, device=self.dev)
-        self.G4 = torch.empty((max(m, 1), 4), dtype=torch.float32, device=self.dev)
-        self.rid = torch.empty(max(m, 1), dtype=torch.int32, device=self.dev)
+        if n > 0:
+            # the survivor count, on its way to the host while the rest is queued
+            self._m_host.copy_(self.off[n - 1:n], non_blocking=True)
+            self._m_ready.record(main)
         # the previous step's k0 update must have finished before k0 is read
         if self._k0_done is not None:
             main.wait_event(self._k0_done)
             self._k0_done = None
+        # The gather is issued before the survivor count reaches the host, into buffers of
+        # the capacity the previous steps needed (it writes no row past it), so the GPU
+        # is busy while the host waits; a step with more survivors gathers again into
+        # larger buffers.
+        self._gather(n, viewdirs)
+        if k0_step is not None:
+            lin = self.rgbnet.linears()
+            self._packed = mlp_hip.Packed([l.weight.detach() for l in lin],
+                                          self.rgbnet.compute_dtype == torch.bfloat16, st)
+        if not sync:
+            return None
+        m = self._survivors(n)
+        if m > self._rows:
+            self._grow_rows(m)
+            self._gather(n, viewdirs)
+        self._set_rows(m)
+        return m
+
+    def _gather(self, n, viewdirs):
         call("vx_fused_fine_gather", ptr(self._k0), self.k0_geom.data_ptr(),
              self.sdf_geom.data_ptr(), self.box.data_ptr(), self.params.data_ptr(),
              ptr(viewdirs), n, self.cap, self.cfg.view_pe, ptr(self.s4), ptr(self.g4),
-             ptr(self.stats), ptr(self.cnt), ptr(self.off), XSTRIDE, ptr(self.X), ptr(self.P4),
-             ptr(self.G4), ptr(self.rid), None, st)
-        return m
+             ptr(self.stats), ptr(self.cnt), ptr(self.off), XSTRIDE, self._rows,
+             ptr(self._X), ptr(self._P4), ptr(self._G4), ptr(self._rid), None,
+             stream_of(self.sdf))
+
+    def _survivors(self, n):
+        """The survivor count of the last forward (waits for its copy to the host)."""
+        if n == 0:
+            return 0
+        self._m_ready.synchronize()
+        return int(self._m_host[0])
+
+    def _set_rows(self, m):
+        self.m = m
+        self.X, self.P4, self.G4, self.rid = (self._X[:max(m, 1)], self._P4[:max(m, 1)],
+                                              self._G4[:max(m, 1)], self._rid[:max(m, 1)])
+
+    def _grow_rows(self, m):
+        """Survivor-row buffers for at least m rows (with a margin)."""
+        rows = int(m * 1.25) + 1024
+        self._rows = rows
+        self._X = torch.empty((rows, XSTRIDE), dtype=torch.float32, device=self.dev)
+        self._P4 = torch.empty((rows, 4), dtype=torch.float32, device=self.dev)
+        self._G4 = torch.empty((rows, 4), dtype=torch.float32, device=self.dev)
+        self._rid = torch.empty(rows, dtype=torch.int32, device=self.dev)
 
     def step(self, rays_o, rays_d, viewdirs, target, profile=False):
         """One training step.  With overlap_adam the k0 update may still be running on the
@@ -253,7 +296,6 @@ class FusedFineTrainer:
         self._parity ^= 1
         k0_step = self.adam["k0"][2] + 1
         self.adam["k0"][2] = k0_step
-        m = self._forward(rays_o, rays_d, viewdirs, pages, k0_step)
         st = stream_of(self.sdf)
         main = torch.cuda.current_stream(self.dev)
         din = cfg.mlp_in
@@ -261,26 +303,44 @@ class FusedFineTrainer:
         Ws = [l.weight.detach() for l in lin]
         bs = [l.bias.detach() for l in lin]
         bf16 = self.rgbnet.compute_dtype == torch.bfloat16
-        if m > 0:
-            packed = mlp_hip.Packed(Ws, bf16, st)
-            Xr = mlp_hip.padded_rows(self.X[:m], din, bf16)
-            lg, hs = mlp_hip.forward_rows(Xr, Ws, bs, bf16, packed, st)
-        else:
-            lg = torch.zeros((1, 3), dtype=torch.float32, device=self.dev)
-        dlogit = torch.empty((max(m, 1), 3), dtype=torch.float32, device=self.dev)
-        gw = torch.empty(max(m, 1), dtype=torch.float32, device=self.dev)
-        call("vx_fused_fine_composite", n, ptr(self.cnt), ptr(self.off), ptr(lg), 3, ptr(self.P4),
-             ptr(self.last), ptr(target), float(cfg.bg), ptr(self.rgb), ptr(dlogit), ptr(gw),
-             ptr(self.glast), ptr(self.S), ptr(self.loss_r), st)
-        if m > 0:
-            dX = mlp_hip.backward_rows(Xr, hs, Ws, bf16, packed, dlogit[:m], st, self.mlp_grads)
-        else:
-            # no survivor: every rgbnet gradient is zero (the buffer is cleared by Adam)
-            dX = torch.zeros((1, XSTRIDE), dtype=torch.float32, device=self.dev)
-        dgf = torch.empty((max(m, 1), 3), dtype=torch.float32, device=self.dev)
-        call("vx_fused_fine_feat_bwd", self.k0_geom.data_ptr(), self.sdf_geom.data_ptr(),
-             self.box.data_ptr(), self.params.data_ptr(), m, ptr(dX), dX.stride(0), ptr(self.P4),
-             ptr(self.G4), ptr(self.k0_grad), ptr(dgf), None, st)
+        self._forward(rays_o, rays_d, viewdirs, pages, k0_step, sync=False)
+        # The colour side is queued with the survivor count still on the device (off[n-1]):
+        # every kernel reads it there and processes min(count, capacity) rows, so the GPU
+        # runs on while the host waits for the count.  Should the count exceed the row
+        # buffers' capacity, the feature gradient written so far (rays inside the
+        # capacity) is cleared and the colour side runs again on larger buffers.
+        m_dev = self.off[n - 1:n] if n > 0 else None
+        while True:
+            rows = self._rows
+            if n > 0:
+                Xr = mlp_hip.padded_rows(self._X, din, bf16)
+                lg, hs = mlp_hip.forward_rows(Xr, Ws, bs, bf16, self._packed, st, M_dev=m_dev)
+            else:
+                lg = torch.zeros((1, 3), dtype=torch.float32, device=self.dev)
+            dlogit = torch.empty((max(rows, 1), 3), dtype=torch.float32, device=self.dev)
+            gw = torch.empty(max(rows, 1), dtype=torch.float32, device=self.dev)
+            call("vx_fused_fine_composite", n, ptr(self.cnt), ptr(self.off), ptr(lg), 3,
+                 ptr(self._P4), ptr(self.last), ptr(target), float(cfg.bg), ptr(self.rgb),
+                 ptr(dlogit), ptr(gw), ptr(self.glast), ptr(self.S), ptr(self.loss_r), rows, st)
+            if n > 0:
+                dX = mlp_hip.backward_rows(Xr, hs, Ws, bf16, self._packed, dlogit, st,
+                                           self.mlp_grads, M_dev=m_dev)
+            else:
+                # no ray: every rgbnet gradient is zero
+                self.mlp_gflat.zero_()
+                dX = torch.zeros((1, XSTRIDE), dtype=torch.float32, device=self.dev)
+            dgf = torch.empty((max(rows, 1), 3), dtype=torch.float32, device=self.dev)
+            call("vx_fused_fine_feat_bwd", self.k0_geom.data_ptr(), self.sdf_geom.data_ptr(),
+                 self.box.data_ptr(), self.params.data_ptr(), rows if n > 0 else 0, ptr(dX),
+                 dX.stride(0), ptr(self._P4), ptr(self._G4), ptr(self.k0_grad), ptr(dgf), None,
+                 ptr(m_dev), st)
+            m = self._survivors(n)
+            if m <= rows:
+                break
+            self.k0_grad.zero_()
+            self._grow_rows(m)
+            self._gather(n, viewdirs)
+        self._set_rows(m)
         if profile:
             # snapshot before any optimiser pass clears a gradient
             k0_grad_snapshot = self.k0_grad.clone()
@@ -307,7 +367,7 @@ class FusedFineTrainer:
         call("vx_fused_fine_bwd", ptr(self.sdf), self.sdf_geom.data_ptr(), self.box.data_ptr(),
              self.params.data_ptr(), ptr(viewdirs), n, self.cap, ptr(self.s4), ptr(self.g4),
              ptr(self.stats), ptr(self.cnt), ptr(self.off), ptr(self.last), ptr(gw),
-             ptr(self.glast), ptr(self.S), ptr(dgf), ptr(self.sdf_grad), st)
+             ptr(self.glast), ptr(self.S), ptr(dgf), ptr(self.sdf_grad), rows, st)
         tv_loss = self._regularise(n, k0_step) if tv_active(self.tv, k0_step) else None
         if profile:
             self.grads_snapshot = dict(sdf=self.sdf_grad.clone(), k0=k0_grad_snapshot,

@@ -51,9 +51,10 @@ class Packed:
         self.kt0 = kt0
 
 
-def forward_rows(X, Ws, bs, bf16, packed, st):
+def forward_rows(X, Ws, bs, bf16, packed, st, M_dev=None):
     """logits [M, 3] and the hidden activations of X [M, 32*kt0] (padded rows in the
-    compute element type)."""
+    compute element type).  M_dev: optional device int64 row count (at most M rows of
+    the buffers are then used), so the launches need not wait for it on the host."""
     M = X.shape[0]
     dev = X.device
     adt = torch.bfloat16 if bf16 else torch.float32
@@ -66,15 +67,16 @@ def forward_rows(X, Ws, bs, bf16, packed, st):
         H = torch.empty((M, WIDTH), dtype=adt, device=dev)
         call("vx_mlp_layer_fwd", int(bf16), kt, int(last), ptr(h_in), h_in.shape[1], M,
              ptr(packed.fwd[li]), ptr(bs[li]), ptr(H), WIDTH, ptr(Ws[-1]) if last else None,
-             ptr(bs[-1]) if last else None, ptr(logits) if last else None, st)
+             ptr(bs[-1]) if last else None, ptr(logits) if last else None, ptr(M_dev), st)
         hs.append(H)
         h_in, kt = H, 6
     return logits, hs
 
 
-def backward_rows(X, hs, Ws, bf16, packed, g, st, grads):
+def backward_rows(X, hs, Ws, bf16, packed, g, st, grads, M_dev=None):
     """Gradients of every weight and bias into the tensors of `grads` ([W0, b0, W1, ...],
-    overwritten) and the gradient of X [M, 32*kt0] (returned, float32)."""
+    overwritten) and the gradient of X [M, 32*kt0] (returned, float32); M_dev as for
+    forward_rows."""
     M = X.shape[0]
     dev = X.device
     nlin = len(Ws)
@@ -83,7 +85,8 @@ def backward_rows(X, hs, Ws, bf16, packed, g, st, grads):
     # output layer (K = 3) and the gradient of the last hidden activations
     dH = torch.empty((M, WIDTH), dtype=torch.float32, device=dev)
     call("vx_mlp_last_bwd", int(bf16), ptr(g), ptr(hs[-1]), WIDTH, M, ptr(Ws[-1]), ptr(dH),
-         WIDTH, ptr(part), ptr(partb), ptr(grads[2 * (nlin - 1)]), ptr(grads[2 * nlin - 1]), st)
+         WIDTH, ptr(part), ptr(partb), ptr(grads[2 * (nlin - 1)]), ptr(grads[2 * nlin - 1]),
+         ptr(M_dev), st)
     dX = None
     for li in range(nlin - 2, -1, -1):
         h_prev = hs[li - 1] if li > 0 else X
@@ -91,17 +94,17 @@ def backward_rows(X, hs, Ws, bf16, packed, g, st, grads):
         # weight gradient of layer li: dY = dH (its output grad), X = its input
         call("vx_mlp_wgrad", int(bf16), kt, Ws[li].shape[1], ptr(dH), WIDTH, ptr(h_prev),
              h_prev.shape[1], M, ptr(part), ptr(partb), ptr(grads[2 * li]),
-             ptr(grads[2 * li + 1]), st)
+             ptr(grads[2 * li + 1]), ptr(M_dev), st)
         # data gradient through layer li
         if li > 0:
             dprev = torch.empty((M, WIDTH), dtype=torch.float32, device=dev)
             call("vx_mlp_layer_bwd", int(bf16), kt, ptr(dH), WIDTH, M, ptr(packed.bwd[li]),
-                 ptr(h_prev), WIDTH, ptr(dprev), WIDTH, st)
+                 ptr(h_prev), WIDTH, ptr(dprev), WIDTH, ptr(M_dev), st)
             dH = dprev
         else:
             dX = torch.empty((M, 32 * kt), dtype=torch.float32, device=dev)
             call("vx_mlp_layer_bwd", int(bf16), kt, ptr(dH), WIDTH, M, ptr(packed.bwd[0]), None,
-                 0, ptr(dX), 32 * kt, st)
+                 0, ptr(dX), 32 * kt, ptr(M_dev), st)
     return dX
 
 

@@ -113,14 +113,14 @@ class ResidentScene:
         rid = torch.empty(max(m, 1), dtype=torch.int32, device=self.dev)
         call("vx_fused_fine_gather", ptr(self.k0), self.k0_geom.data_ptr(), self.sdf_geom.data_ptr(),
              self.box.data_ptr(), self.params.data_ptr(), ptr(viewdirs), n, self.cap, self.cfg.view_pe,
-             ptr(ws.s4), ptr(ws.g4), ptr(ws.stats), ptr(ws.cnt), ptr(ws.off), XSTRIDE, ptr(X),
+             ptr(ws.s4), ptr(ws.g4), ptr(ws.stats), ptr(ws.cnt), ptr(ws.off), XSTRIDE, m, ptr(X),
              ptr(P4), ptr(G4), ptr(rid), None, st)
         if m > 0:
             logits = self.rgbnet.forward_padded(X[:m]).contiguous()
         else:
             logits = torch.zeros((1, 3), dtype=torch.float32, device=self.dev)
         call("vx_fused_fine_composite", n, ptr(ws.cnt), ptr(ws.off), ptr(logits), 3, ptr(P4),
-             ptr(ws.last), None, float(bg), ptr(out), None, None, None, None, None, st)
+             ptr(ws.last), None, float(bg), ptr(out), None, None, None, None, None, m, st)
         return out
 
     @torch.no_grad()

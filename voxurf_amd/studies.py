@@ -260,7 +260,8 @@ def _render_with(t, o, d, v):
         torch.zeros((1, 3), device=o.device)
     out = torch.empty((n, 3), device=o.device)
     call("vx_fused_fine_composite", n, ptr(t.cnt), ptr(t.off), ptr(logits), 3, ptr(t.P4),
-         ptr(t.last), None, float(t.cfg.bg), ptr(out), None, None, None, None, None, stream_of(o))
+         ptr(t.last), None, float(t.cfg.bg), ptr(out), None, None, None, None, None, m,
+         stream_of(o))
     return out
 
 
