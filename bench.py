@@ -34,7 +34,10 @@ def parse():
                          "timing (untimed, no training): the Adam sweep of a short run then "
                          "covers what a long run's would")
     ap.add_argument("--path", choices=["fused", "ops"], default="fused")
-    ap.add_argument("--mlp", choices=["fp32", "bf16"], default="fp32")
+    ap.add_argument("--mlp", choices=["fp32", "fp32-mfma", "bf16"], default="fp32",
+                    help="rgbnet arithmetic: fp32 = FP32 accuracy through three-way BF16 "
+                         "splits (six BF16 MFMA products per pair); fp32-mfma = FP32 MFMAs; "
+                         "bf16 = single BF16 MFMAs, bf16 activations")
     ap.add_argument("--rays", type=int, default=8192)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serial-adam", action="store_true",
@@ -121,8 +124,10 @@ def main():
     if args.path == "fused":
         trainer = FusedFineTrainer(cfg, sdf, k0, mlp, device=dev, mlp_dtype=mlp_dtype)
         trainer.overlap_adam = not args.serial_adam
+        trainer.rgbnet.f32_split = args.mlp != "fp32-mfma"
     else:
         model = VoxurfFine(cfg, sdf, k0, mlp, mlp_dtype=mlp_dtype).to(dev)
+        model.rgbnet.f32_split = args.mlp != "fp32-mfma"
         trainer = Trainer(model, cfg)
     del sdf, k0
 
@@ -207,7 +212,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if args.mlp == "fp32" else "f32 (rgbnet bf16)",
+        "dtype": "f32 (rgbnet bf16)" if args.mlp == "bf16" else "f32",
         "data": "synthetic (seeded analytic SDF scene, random-init features and rgbnet)",
         "config": {"workload": "plain fine step: 256^3 SDF + 256^3x12 features, rgbnet "
                                "42-192x3-3, 8192 rays, stepsize 0.5, s=256, fwd+bwd+Adam",

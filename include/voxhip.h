@@ -403,18 +403,25 @@ int vx_fused_fine_bwd(const float* sdf, const int64_t* sdf_geom, const float* bo
                       int64_t max_rows, void* stream);
 /* ---- rgbnet on the matrix cores ---------------------------------------------------
  * The colour MLP of lib/voxurf_fine.py:132-149 (nn.Linear/ReLU layers of width 192,
- * which the reference runs as cuBLAS GEMMs plus elementwise kernels).  bf16 = 0: FP32
- * MFMAs; bf16 = 1: BF16 MFMAs with FP32 accumulation and BF16 activations.
+ * which the reference runs as cuBLAS GEMMs plus elementwise kernels).  prec selects
+ * the arithmetic:
+ *   0: FP32 MFMAs, float32 activations;
+ *   1: BF16 MFMAs with FP32 accumulation, bfloat16 activations;
+ *   2: FP32 accuracy from BF16 MFMAs: both operands of every product are split into three
+ *      bfloat16 parts (x = x0 + x1 + x2 exactly) and the six part products with
+ *      i + j <= 2 accumulate in FP32, within 2^-23 relative of the exact product;
+ *      float32 activations (the weight-gradient GEMMs run as with prec 0).
  * Weights are packed per call into the MFMA operand order; activation rows are
- * [M, 192] (element type per bf16), inputs [M, 32*kt] zero padded.  M_dev (may be NULL):
+ * [M, 192] (element type per prec), inputs [M, 32*kt] zero padded.  M_dev (may be NULL):
  * a device int64 row count; when given the calls process min(*M_dev, M) rows, M being
  * the buffers' capacity, so they can be queued before the count is known on the host. */
 
 /* Pack W [n_out, k_in] (row-major, float32) as the A operand of a layer with kt input
  * tiles of 32 (reduction) and nto output tiles; transpose = 1 packs W^T (backward). */
-int vx_mlp_pack(int bf16, const float* W, int n_out, int k_in, int kt, int nto, int transpose,
+int vx_mlp_pack(int prec, const float* W, int n_out, int k_in, int kt, int nto, int transpose,
                 void* out, void* stream);
-/* Element count of a packed operand. */
+/* Element count of a packed operand (float32 for prec 0, bfloat16 for prec 1; prec 2
+ * packs three times as many bfloat16 elements). */
 int64_t vx_mlp_pack_elems(int kt, int nto);
 /* All packed operands of an rgbnet in one launch: w_ptrs is a host array of the
  * n_hidden hidden layers' device weight pointers (layer 0 [192, k_in0], the others
@@ -422,15 +429,15 @@ int64_t vx_mlp_pack_elems(int kt, int nto);
  * 0 -- k_in0 <= 64 or <= 96 -- else 6; nto = 6) then the transposed backward operand
  * (kt = 6, nto = the forward kt), concatenated. */
 int64_t vx_mlp_pack_net_elems(int n_hidden, int k_in0);
-int vx_mlp_pack_net(int bf16, int n_hidden, const int64_t* w_ptrs, int k_in0, void* out,
+int vx_mlp_pack_net(int prec, int n_hidden, const int64_t* w_ptrs, int k_in0, void* out,
                     void* stream);
 /* One hidden layer: out = relu(W in + b); with last = 1 also logits [M,3] = W3 out + b3. */
-int vx_mlp_layer_fwd(int bf16, int kt, int last, const void* in, int in_stride, int64_t M,
+int vx_mlp_layer_fwd(int prec, int kt, int last, const void* in, int in_stride, int64_t M,
                      const void* wpk, const float* bias, void* out, int out_stride,
                      const float* w3, const float* b3, float* logits, const int64_t* M_dev,
                      void* stream);
 /* Data gradient through a layer: dx = (W^T dy) * [mask > 0] (mask NULL: no ReLU). */
-int vx_mlp_layer_bwd(int bf16, int nto, const float* dy, int dy_stride, int64_t M,
+int vx_mlp_layer_bwd(int prec, int nto, const float* dy, int dy_stride, int64_t M,
                      const void* wpk_t, const void* mask, int mask_stride, float* dx,
                      int dx_stride, const int64_t* M_dev, void* stream);
 /* Floats the `part` / `partb` workspaces of vx_mlp_wgrad and vx_mlp_last_bwd need for
@@ -439,12 +446,12 @@ int64_t vx_mlp_wgrad_part_elems(int64_t M);
 int64_t vx_mlp_wgrad_partb_elems(int64_t M);
 /* dW [192, k_real] = dY^T X, db [192] = column sums of dY (split over row blocks, then
  * a deterministic two-pass reduction). */
-int vx_mlp_wgrad(int bf16, int kt, int k_real, const float* dY, int dys, const void* X, int xs,
+int vx_mlp_wgrad(int prec, int kt, int k_real, const float* dY, int dys, const void* X, int xs,
                  int64_t M, float* part, float* partb, float* dW, float* db, const int64_t* M_dev,
                  void* stream);
 /* Backward through the 3-output layer in one pass over H: dH = (dlogit W3) * [H > 0],
  * dW3 [3, 192] = dlogit^T H, db3 [3] = column sums of dlogit. */
-int vx_mlp_last_bwd(int bf16, const float* dlogit, const void* H, int hs, int64_t M,
+int vx_mlp_last_bwd(int prec, const float* dlogit, const void* H, int hs, int64_t M,
                     const float* w3, float* dH, int dhs, float* part, float* partb, float* dW3,
                     float* db3, const int64_t* M_dev, void* stream);
 /* y = bfloat16(x), n elements. */

@@ -1,7 +1,11 @@
 """rgbnet on the matrix-core kernels vs nn.Linear chains (GPU).
 
-FP32 path: logits within 1e-5 relative of a float64 CPU reference (floor 1e-6 of the
-batch maximum), input/weight/bias gradients within 1e-4 relative (floor 1e-5).
+FP32 paths (FP32 MFMAs, and FP32 through three-way BF16 splits): logits within 1e-5
+relative of a float64 CPU reference (floor 1e-6 of the batch maximum), input/weight/bias
+gradients within 1e-4 relative (floor 1e-5) of a float64 backward pass through the
+kernels' own ReLU masks (ref_f64); the split path's largest errors are also
+held to those of the FP32-MFMA path on the same network (a dropped or misplaced part
+would cost 2^-8 relative).
 BF16 path (BF16 MFMA operands, FP32 accumulation, BF16 stored activations): against a
 float64 emulation that rounds to bfloat16 at exactly the points the kernels do (the
 network input, the packed weights, each stored activation, the incoming gradient of each
@@ -14,6 +18,8 @@ import pytest
 import torch
 import torch.nn as nn
 
+from voxurf_amd import mlp_hip
+from voxurf_amd._lib import stream_of
 from voxurf_amd.mlp import RgbNet
 
 pytestmark = pytest.mark.gpu
@@ -27,6 +33,46 @@ def ref_forward(x, net):
         if i < len(lins) - 1:
             h = torch.relu(h)
     return h
+
+
+def gpu_masks(net, x, prec):
+    """ReLU masks [M, 192] per hidden layer as the kernels evaluate them (their own
+    activations > 0; the kernels are deterministic, so these are the ones the autograd
+    pass used)."""
+    lins = net.linears()
+    Ws = [l.weight.detach().contiguous() for l in lins]
+    bs = [l.bias.detach().contiguous() for l in lins]
+    st = stream_of(x)
+    X = mlp_hip.padded_rows(x.detach(), lins[0].in_features, prec)
+    _, hs = mlp_hip.forward_rows(X, Ws, bs, prec, mlp_hip.Packed(Ws, prec, st), st)
+    return [(h > 0).cpu() for h in hs]
+
+
+def ref_f64(x, net, g, masks):
+    """float64 logits, input gradient and parameter gradients; the backward pass takes
+    the ReLU masks given (where a pre-activation lies within FP32 rounding of zero, the
+    kernels' mask may differ from float64's, which moves that row's gradient by a whole
+    unit's contribution; a float64 pass through the same masks is then the reference)."""
+    lins = net.linears()
+    W = [l.weight.detach().double().cpu() for l in lins]
+    b = [l.bias.detach().double().cpu() for l in lins]
+    h = x.detach().double().cpu()
+    g = g.double().cpu()
+    hs = [h]
+    for i in range(len(lins) - 1):
+        h = torch.relu(h @ W[i].T + b[i])
+        hs.append(h)
+    logits = h @ W[-1].T + b[-1]
+    grads = [None] * (2 * len(lins))
+    grads[-2], grads[-1] = g.T @ hs[-1], g.sum(0)
+    dh = (g @ W[-1]) * masks[-1]
+    for li in range(len(lins) - 2, -1, -1):
+        grads[2 * li], grads[2 * li + 1] = dh.T @ hs[li], dh.sum(0)
+        d = dh @ W[li]
+        if li > 0:
+            d = d * masks[li - 1]
+        dh = d
+    return logits, dh, grads
 
 
 def bf(t):
@@ -62,11 +108,13 @@ def rel_err(a, b):
     return float((a.double().cpu() - b).abs().max() / (b.abs().max() + 1e-30))
 
 
-@pytest.mark.parametrize("din,M", [(42, 5000), (42, 31), (79, 4099), (42, 0)])
-@pytest.mark.parametrize("dtype", [torch.float32, torch.bfloat16])
-def test_rgbnet_mfma(dev, din, M, dtype):
-    torch.manual_seed(0)
+def run_net(dev, din, M, dtype, f32_split, seed=0):
+    """(net, x, out, g, float64 logits, [(name, grad, float64 grad)]); the float64
+    gradients go through the kernels' ReLU masks for the FP32 paths (ref_f64), through
+    float64's own for BF16."""
+    torch.manual_seed(seed)
     net = RgbNet(din, 192, 4, compute_dtype=dtype).to(dev)
+    net.f32_split = f32_split
     with torch.no_grad():
         for l in net.linears():
             l.bias.uniform_(-0.1, 0.1)
@@ -74,25 +122,39 @@ def test_rgbnet_mfma(dev, din, M, dtype):
     out = net(x)
     g = torch.randn(M, 3, device=dev)
     out.backward(g)
-    # float64 reference on the CPU
-    ref = RgbNet(din, 192, 4).double()
-    ref.load_state_dict({k: v.double().cpu() for k, v in net.state_dict().items()})
-    xr = x.detach().double().cpu().requires_grad_(True)
-    o_r = ref_forward(xr, ref)
-    o_r.backward(g.double().cpu())
+    names = ["x"] + [n for n, _ in net.named_parameters()]
+    got = [x.grad] + [p.grad for p in net.parameters()]
+    if dtype == torch.float32 and M > 0:
+        masks = gpu_masks(net, x, mlp_hip.prec_of(dtype, f32_split))
+        o_r, dx, grads = ref_f64(x, net, g, masks)
+        want = [dx[:, :din]] + grads
+    else:
+        ref = RgbNet(din, 192, 4).double()
+        ref.load_state_dict({k: v.double().cpu() for k, v in net.state_dict().items()})
+        xr = x.detach().double().cpu().requires_grad_(True)
+        o_r = ref_forward(xr, ref)
+        o_r.backward(g.double().cpu())
+        o_r = o_r.detach()
+        want = [xr.grad] + [q.grad for q in ref.parameters()]
+    return net, x, out, g, o_r, list(zip(names, got, want))
+
+
+@pytest.mark.parametrize("din,M", [(42, 5000), (42, 31), (79, 4099), (42, 0)])
+@pytest.mark.parametrize("prec", ["f32_split", "f32", "bf16"])
+def test_rgbnet_mfma(dev, din, M, prec):
+    dtype = torch.bfloat16 if prec == "bf16" else torch.float32
+    net, x, out, g, o_r, pairs = run_net(dev, din, M, dtype, prec == "f32_split")
     if M == 0:
         assert out.shape == (0, 3)
         return
-    pairs = [("x", x.grad, xr.grad)] + [
-        (n, p.grad, q.grad) for (n, p), q in zip(net.named_parameters(), ref.parameters())]
     if dtype == torch.float32:
-        torch.testing.assert_close(out.double().cpu(), o_r.detach(), rtol=1e-5,
+        torch.testing.assert_close(out.double().cpu(), o_r, rtol=1e-5,
                                    atol=1e-6 * float(o_r.abs().max()))
         for n, a, b in pairs:
             torch.testing.assert_close(a.double().cpu(), b, rtol=1e-4, atol=1e-5 * float(b.abs().max()),
                                        msg=lambda m: f"{n}: {m}")
     else:
-        e = rel_err(out, o_r.detach())
+        e = rel_err(out, o_r)
         print(f"bf16 logits vs float64: rel err {e:.3g}")
         assert e < 3e-2
         for n, a, b in pairs:
@@ -106,3 +168,19 @@ def test_rgbnet_mfma(dev, din, M, dtype):
             e = rel_err(a, b)
             print(f"bf16 grad {n} vs emulation: rel err {e:.3g}")
             assert e < 5e-3, n
+
+
+@pytest.mark.parametrize("din,M", [(42, 20000), (79, 3000)])
+def test_split_as_accurate_as_fp32_mfma(dev, din, M):
+    """Largest error against float64 of every output and gradient: the BF16-split path
+    within 8x that of the FP32-MFMA path (plus 1e-7 of the entry's scale) on the same
+    network and inputs."""
+    _, _, out_n, _, o_r, pairs_n = run_net(dev, din, M, torch.float32, False, seed=5)
+    _, _, out_s, _, _, pairs_s = run_net(dev, din, M, torch.float32, True, seed=5)
+    e_n, e_s = rel_err(out_n, o_r), rel_err(out_s, o_r)
+    print(f"logits: split {e_s:.3g}, fp32 mfma {e_n:.3g}")
+    assert e_s <= 8 * e_n + 1e-7
+    for (n, a_n, b_n), (_, a_s, b_s) in zip(pairs_n, pairs_s):
+        e_n, e_s = rel_err(a_n, b_n), rel_err(a_s, b_s)
+        print(f"grad {n}: split {e_s:.3g}, fp32 mfma {e_n:.3g}")
+        assert e_s <= 8 * e_n + 1e-7, n

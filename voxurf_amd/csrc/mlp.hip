@@ -81,6 +81,48 @@ __device__ __forceinline__ float pack_bf16_value(const float* __restrict__ Wt, i
   return v;
 }
 
+// ---- FP32 through three-way BF16 splits --------------------------------------------
+// A float32 x is the exact sum of three bfloat16 values: x0 = bf16(x) (round to nearest
+// even), x1 = bf16(x - x0), x2 = x - x0 - x1 (each difference is exact in float32 by
+// Sterbenz's lemma, and x2 has at most 8 significant bits).  A product x w is then the
+// sum of the nine x_i w_j; the six with i + j <= 2 are computed (BF16 MFMAs, each
+// product exact in the FP32 accumulator) and the three dropped terms are below
+// 2^-23 |x w| together.  One 16-deep reduction step takes six 32x32x16 BF16 MFMAs
+// (6 x 32 cycles) against eight 32x32x2 FP32 MFMAs (8 x 64 cycles).
+__device__ __forceinline__ void split3(float x, __bf16& p0, __bf16& p1, __bf16& p2) {
+  p0 = (__bf16)x;
+  const float r1 = x - (float)p0;
+  p1 = (__bf16)r1;
+  p2 = (__bf16)(r1 - (float)p1);
+}
+
+__device__ __forceinline__ void split8(const float4& a, const float4& b, bf16x8& p0, bf16x8& p1,
+                                       bf16x8& p2) {
+  const float x[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    __bf16 q0, q1, q2;
+    split3(x[e], q0, q1, q2);
+    p0[e] = q0;
+    p1[e] = q1;
+    p2[e] = q2;
+  }
+}
+
+// Packed A operand, split: the BF16 operand order (pack_bf16_value) with the three
+// parts of each step side by side: bf16x8 vector ((to * S + s) * 3 + part) * 64 + lane.
+template <typename Seg>
+__device__ __forceinline__ void pack_split_store(const Seg& sg, int64_t idx, __bf16* out) {
+  const float v = pack_bf16_value(sg.W, sg.n_out, sg.k_in, sg.kt, sg.transpose, idx);
+  const int64_t e = idx & 7, lane = (idx >> 3) & 63, ts = idx >> 9;  // ts = to * S + s
+  __bf16 q0, q1, q2;
+  split3(v, q0, q1, q2);
+  __bf16* o = out + 3 * sg.offset + ((ts * 3) * 64 + lane) * 8 + e;
+  o[0] = q0;
+  o[64 * 8] = q1;
+  o[2 * 64 * 8] = q2;
+}
+
 __global__ void k_pack_bf16(const float* __restrict__ Wt, int n_out, int k_in, int KT, int NTO,
                             int transpose, __bf16* __restrict__ out) {
   const int64_t total = (int64_t)NTO * 2 * KT * 64 * 8;
@@ -100,13 +142,16 @@ struct PackNet {
   PackSeg seg[kMaxPackSegs];
 };
 
-template <bool BF>
+// PREC 0: FP32 operands; 1: BF16; 2: three BF16 parts per value (pack_split_store).
+template <int PREC>
 __global__ void k_pack_net(PackNet net, void* __restrict__ out) {
   const PackSeg sg = net.seg[blockIdx.y];
-  const int64_t total = (int64_t)sg.nto * 16 * sg.kt * 64;  // same count in both layouts
+  const int64_t total = (int64_t)sg.nto * 16 * sg.kt * 64;  // same count in every layout
   for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < total;
        idx += (int64_t)gridDim.x * blockDim.x) {
-    if constexpr (BF) {
+    if constexpr (PREC == 2) {
+      pack_split_store(sg, idx, reinterpret_cast<__bf16*>(out));
+    } else if constexpr (PREC == 1) {
       reinterpret_cast<__bf16*>(out)[sg.offset + idx] =
           (__bf16)pack_bf16_value(sg.W, sg.n_out, sg.k_in, sg.kt, sg.transpose, idx);
     } else {
@@ -343,6 +388,244 @@ __global__ void __launch_bounds__(kLayerThreads) k_mlp_layer(
       layer_tile<KT, NTO, MODE, BF, false>(in_, in_stride, M, lds_raw, lb, mask_, mask_stride, out_,
                                            out_stride, tile, lo, hi);
     u += hi - lo;
+  }
+}
+
+// ---- one layer, FP32 through BF16 splits --------------------------------------------
+// The modes of k_mlp_layer with float32 inputs, outputs and masks, the inputs split into
+// three BF16 parts as they are loaded and the weights packed in three parts
+// (pack_split_store).  The block's weights cover NTL of the layer's NTL * NCG output
+// column tiles: with NCG = 2 (192 outputs, 110.6 KB of weights in LDS) block b serves
+// column group (b / 8) % 2, so blocks b and b + 8, which run on the same XCD, read the
+// same input rows and the second read is an L2 hit; the blocks of a group split its
+// ntiles x NTL output blocks evenly over their waves (stream-K).
+//
+// A 32-feature slab of a lane's row is four float4 loads; one slab feeds 2 * 6 * NTL
+// MFMAs, about a microsecond, less than a loaded HBM round trip, so slabs are loaded two
+// ahead (three in registers), and the last two slabs of a tile prefetch the first two of
+// the wave's next tile.
+
+// The two 16-deep reduction steps of slab t (features 32 t + 16 q + 4 h + {0..3} and
+// the same + 8 for step 2 t + q) into the accumulators of output tiles [to_lo, to_hi).
+template <int KT, int NTL, bool FULL>
+__device__ __forceinline__ void split_slab(f32x16 (&acc)[NTL], const bf16x8* lw, int lane,
+                                           const float4 (&x)[4], int t, int to_lo, int to_hi) {
+  constexpr int S = 2 * KT;
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    bf16x8 b0, b1, b2;
+    split8(x[2 * q], x[2 * q + 1], b0, b1, b2);
+    bf16x8 a0[NTL], a1[NTL], a2[NTL];
+#pragma unroll
+    for (int to = 0; to < NTL; ++to) {
+      if (!FULL && (to < to_lo || to >= to_hi)) continue;
+      const bf16x8* a = lw + ((to * S + 2 * t + q) * 3) * 64 + lane;
+      a0[to] = a[0];
+      a1[to] = a[64];
+      a2[to] = a[128];
+    }
+    // smallest terms first; independent accumulators interleaved
+#define SPLIT_MFMA(A, B)                                                              \
+  _Pragma("unroll") for (int to = 0; to < NTL; ++to) {                                \
+    if (!FULL && (to < to_lo || to >= to_hi)) continue;                               \
+    acc[to] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[to], B, acc[to], 0, 0, 0);    \
+  }
+    SPLIT_MFMA(a2, b0)
+    SPLIT_MFMA(a1, b1)
+    SPLIT_MFMA(a0, b2)
+    SPLIT_MFMA(a1, b0)
+    SPLIT_MFMA(a0, b1)
+    SPLIT_MFMA(a0, b0)
+#undef SPLIT_MFMA
+  }
+}
+
+__device__ __forceinline__ void load_slab(float4 (&x)[4], const float* __restrict__ in,
+                                          int in_stride, int64_t row, int t, int h) {
+  const float* rp = in + row * in_stride + 32 * t + 4 * h;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) x[u] = *reinterpret_cast<const float4*>(rp + 8 * u);
+}
+
+// Epilogue of k_mlp_layer's modes on float32 rows: bias and ReLU (MODE 0), the ReLU mask
+// (MODE 2), or the plain product (MODE 3); columns col0 + 32 to + ...
+template <int NTL, int MODE, bool FULL>
+__device__ __forceinline__ void split_store(const f32x16 (&acc)[NTL], const float* lb,
+                                            const float* __restrict__ mask, int mask_stride,
+                                            float* __restrict__ out, int out_stride, int64_t row,
+                                            int h, int to_lo, int to_hi, int col0) {
+  const int64_t obase = row * out_stride + 4 * h;
+  const int64_t mbase = row * mask_stride + 4 * h;
+#pragma unroll
+  for (int to = 0; to < NTL; ++to) {
+    if (!FULL && (to < to_lo || to >= to_hi)) continue;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      float v4[4];
+#pragma unroll
+      for (int w = 0; w < 4; ++w) {
+        const int v = 4 * u + w;
+        float x = acc[to][v];
+        if (MODE == 0) {
+          x += lb[dfeat(to, v, h)];
+          x = x > 0.f ? x : 0.f;
+        }
+        v4[w] = x;
+      }
+      const int cofs = col0 + 32 * to + 8 * u;
+      if (MODE == 2) {
+        const float4 mk = *reinterpret_cast<const float4*>(mask + mbase + cofs);
+        v4[0] = mk.x > 0.f ? v4[0] : 0.f;
+        v4[1] = mk.y > 0.f ? v4[1] : 0.f;
+        v4[2] = mk.z > 0.f ? v4[2] : 0.f;
+        v4[3] = mk.w > 0.f ? v4[3] : 0.f;
+      }
+      *reinterpret_cast<float4*>(out + obase + cofs) = make_float4(v4[0], v4[1], v4[2], v4[3]);
+    }
+  }
+}
+
+// One 32-row tile, output tiles [to_lo, to_hi).  On entry ring[0] and ring[1] hold (or
+// are loading) slabs 0 and 1 of this tile; slab t + 2 goes into ring[(t + 2) % 3] while
+// slab t is multiplied, from the next tile (next_row: this lane's row in it) once t + 2
+// passes the end.  KT % 3 == 0 keeps the ring positions the same for every tile.
+template <int KT, int NTL, int MODE, bool FULL>
+__device__ __forceinline__ void split_tile(float4 (&ring)[3][4], const float* __restrict__ in,
+                                           int in_stride, int64_t M, const bf16x8* lw,
+                                           const float* lb, const float* __restrict__ mask,
+                                           int mask_stride, float* __restrict__ out,
+                                           int out_stride, int64_t tile, int64_t next_row,
+                                           int to_lo, int to_hi, int col0) {
+  static_assert(KT % 3 == 0, "ring of three slabs");
+  const int lane = threadIdx.x & 63, h = lane >> 5, rl = lane & 31;
+  const int64_t row = tile * 32 + rl;
+  const int64_t rrow = row < M ? row : M - 1;
+  f32x16 acc[NTL];
+#pragma unroll
+  for (int t = 0; t < NTL; ++t)
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[t][v] = 0.f;
+#pragma unroll
+  for (int t = 0; t < KT; ++t) {
+    if (t + 2 < KT) load_slab(ring[(t + 2) % 3], in, in_stride, rrow, t + 2, h);
+    else if (next_row >= 0) load_slab(ring[(t + 2) % 3], in, in_stride, next_row, t + 2 - KT, h);
+    split_slab<KT, NTL, FULL>(acc, lw, lane, ring[t % 3], t, to_lo, to_hi);
+  }
+  if (row < M)
+    split_store<NTL, MODE, FULL>(acc, lb, mask, mask_stride, out, out_stride, row, h, to_lo,
+                                 to_hi, col0);
+}
+
+// The same for KT = 2 (the 64-wide first layer): both slabs of the next tile load while
+// this tile's are multiplied (ring[2..3] <- next tile, then swapped).
+template <int NTL, int MODE, bool FULL>
+__device__ __forceinline__ void split_tile2(float4 (&ring)[4][4], const float* __restrict__ in,
+                                            int in_stride, int64_t M, const bf16x8* lw,
+                                            const float* lb, const float* __restrict__ mask,
+                                            int mask_stride, float* __restrict__ out,
+                                            int out_stride, int64_t tile, int64_t next_row,
+                                            int to_lo, int to_hi, int col0) {
+  const int lane = threadIdx.x & 63, h = lane >> 5, rl = lane & 31;
+  const int64_t row = tile * 32 + rl;
+  f32x16 acc[NTL];
+#pragma unroll
+  for (int t = 0; t < NTL; ++t)
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[t][v] = 0.f;
+  if (next_row >= 0) {
+    load_slab(ring[2], in, in_stride, next_row, 0, h);
+    load_slab(ring[3], in, in_stride, next_row, 1, h);
+  }
+  split_slab<2, NTL, FULL>(acc, lw, lane, ring[0], 0, to_lo, to_hi);
+  split_slab<2, NTL, FULL>(acc, lw, lane, ring[1], 1, to_lo, to_hi);
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    ring[0][u] = ring[2][u];
+    ring[1][u] = ring[3][u];
+  }
+  if (row < M)
+    split_store<NTL, MODE, FULL>(acc, lb, mask, mask_stride, out, out_stride, row, h, to_lo,
+                                 to_hi, col0);
+}
+
+template <int KT, int NTL, int NCG, int MODE>
+__global__ void __launch_bounds__(kLayerThreads) k_mlp_layer_split(
+    const float* __restrict__ in, int in_stride, int64_t M, const int64_t* __restrict__ Md,
+    const bf16x8* __restrict__ wpk, const float* __restrict__ bias,
+    const float* __restrict__ mask, int mask_stride, float* __restrict__ out, int out_stride) {
+  static_assert(KT == 2 || KT % 3 == 0, "input width 64, 96 or 192");
+  constexpr int S = 2 * KT;
+  constexpr int WV = NTL * S * 3 * 64;   // bf16x8 vectors of one column group
+  __shared__ bf16x8 lw[WV];
+  __shared__ float lb[NTL * 32];
+  const int tid = threadIdx.x;
+  const int cg = NCG == 1 ? 0 : (int)((blockIdx.x >> 3) % NCG);
+  const int64_t bw = NCG == 1 ? (int64_t)blockIdx.x
+                              : (int64_t)((blockIdx.x >> 3) / NCG) * 8 + (blockIdx.x & 7);
+  const int64_t nb = gridDim.x / NCG;
+  {
+    const uint4* src = reinterpret_cast<const uint4*>(wpk) + (int64_t)cg * WV;
+    uint4* dst = reinterpret_cast<uint4*>(lw);
+    constexpr int P = (WV + kLayerThreads - 1) / kLayerThreads;
+    uint4 v[P];
+#pragma unroll
+    for (int b = 0; b < P; ++b) {
+      const int i = tid + b * kLayerThreads;
+      v[b] = i < WV ? src[i] : make_uint4(0u, 0u, 0u, 0u);
+    }
+#pragma unroll
+    for (int b = 0; b < P; ++b) {
+      const int i = tid + b * kLayerThreads;
+      if (i < WV) dst[i] = v[b];
+    }
+    if (MODE == 0)
+      for (int i = tid; i < NTL * 32; i += blockDim.x) lb[i] = bias[cg * NTL * 32 + i];
+  }
+  __syncthreads();
+  // the wave index is uniform: the tile and column choices below stay scalar
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lane = tid & 63, h = lane >> 5, rl = lane & 31;
+  M = eff_rows(M, Md);
+  const int64_t ntiles = (M + 31) / 32;
+  const int64_t nw = nb * (blockDim.x >> 6);
+  const int64_t gw = bw * (blockDim.x >> 6) + wave;
+  const int64_t total = ntiles * NTL;
+  int64_t u = total * gw / nw;
+  const int64_t u1 = total * (gw + 1) / nw;
+  const int col0 = cg * NTL * 32;
+  // this lane's row of a tile, clamped into [0, M) (rows past M are computed, not stored)
+  auto lane_row = [&](int64_t tile) {
+    const int64_t r = tile * 32 + rl;
+    return r < M ? r : M - 1;
+  };
+  constexpr int R = KT == 2 ? 4 : 3;
+  float4 ring[R][4];
+  if (u < u1) {
+    load_slab(ring[0], in, in_stride, lane_row(u / NTL), 0, h);
+    load_slab(ring[1], in, in_stride, lane_row(u / NTL), 1, h);
+  }
+  while (u < u1) {
+    const int64_t tile = u / NTL;
+    const int lo = (int)(u - tile * NTL);
+    const int hi = (u1 - u) < (int64_t)(NTL - lo) ? lo + (int)(u1 - u) : NTL;
+    const int64_t un = u + (hi - lo);
+    const int64_t next_row = un < u1 ? lane_row(un / NTL) : -1;
+    if constexpr (KT == 2) {
+      if (lo == 0 && hi == NTL)
+        split_tile2<NTL, MODE, true>(ring, in, in_stride, M, lw, lb, mask, mask_stride, out,
+                                     out_stride, tile, next_row, 0, NTL, col0);
+      else
+        split_tile2<NTL, MODE, false>(ring, in, in_stride, M, lw, lb, mask, mask_stride, out,
+                                      out_stride, tile, next_row, lo, hi, col0);
+    } else {
+      if (lo == 0 && hi == NTL)
+        split_tile<KT, NTL, MODE, true>(ring, in, in_stride, M, lw, lb, mask, mask_stride, out,
+                                        out_stride, tile, next_row, 0, NTL, col0);
+      else
+        split_tile<KT, NTL, MODE, false>(ring, in, in_stride, M, lw, lb, mask, mask_stride, out,
+                                         out_stride, tile, next_row, lo, hi, col0);
+    }
+    u = un;
   }
 }
 
@@ -655,6 +938,25 @@ static void launch_layer(const void* in, int in_stride, int64_t M, const int64_t
       in, in_stride, M, Md, wpk, bias, mask, mask_stride, out, out_stride);
 }
 
+template <int KT, int NTL, int NCG, int MODE>
+static void launch_layer_split(const void* in, int in_stride, int64_t M, const int64_t* Md,
+                               const void* wpk, const float* bias, const void* mask,
+                               int mask_stride, void* out, int out_stride, hipStream_t st) {
+  // per column group: about two 32 x 32 output blocks per wave at least, at most one
+  // block per CU over all groups (the weights fill the LDS); with two groups the group
+  // size is a multiple of 8 (the block -> group map pairs blocks 8 apart)
+  const int64_t tiles = (M + 31) / 32;
+  int64_t b = (tiles * NTL + 15) / 16;
+  const int64_t cap = (int64_t)num_cus() / NCG;
+  if (b > cap) b = cap;
+  if (b < 1) b = 1;
+  if (NCG > 1) b = (b + 7) / 8 * 8;
+  k_mlp_layer_split<KT, NTL, NCG, MODE><<<(unsigned)(b * NCG), kLayerThreads, 0, st>>>(
+      reinterpret_cast<const float*>(in), in_stride, M, Md, reinterpret_cast<const bf16x8*>(wpk),
+      bias, reinterpret_cast<const float*>(mask), mask_stride, reinterpret_cast<float*>(out),
+      out_stride);
+}
+
 // Row blocks of the weight-gradient kernel: one per CU (each holds two row chunks in
 // LDS), at least one chunk of rows per block.
 static int wgrad_blocks(int64_t M, int kt) {
@@ -678,11 +980,16 @@ using namespace vx;
 
 extern "C" {
 
-int vx_mlp_pack(int bf16, const float* W, int n_out, int k_in, int kt, int nto, int transpose,
+int vx_mlp_pack(int prec, const float* W, int n_out, int k_in, int kt, int nto, int transpose,
                 void* out, void* stream) {
+  VX_CHECK_ARG(prec >= 0 && prec <= 2, "mlp_pack: prec must be 0, 1 or 2");
   VX_CHECK_ARG(kt >= 1 && nto >= 1 && n_out >= 1 && k_in >= 1, "mlp_pack: bad sizes");
   hipStream_t st = (hipStream_t)stream;
-  if (bf16)
+  if (prec == 2) {
+    PackNet net{};
+    net.seg[0] = PackSeg{W, 0, n_out, k_in, kt, nto, transpose};
+    k_pack_net<2><<<dim3(grid_for((int64_t)nto * kt * 16 * 64, 256), 1), 256, 0, st>>>(net, out);
+  } else if (prec == 1)
     k_pack_bf16<<<grid_for((int64_t)nto * kt * 2 * 64 * 8, 256), 256, 0, st>>>(
         W, n_out, k_in, kt, nto, transpose, (__bf16*)out);
   else
@@ -725,8 +1032,9 @@ int64_t vx_mlp_pack_net_elems(int n_hidden, int k_in0) {
   return total;
 }
 
-int vx_mlp_pack_net(int bf16, int n_hidden, const int64_t* w_ptrs, int k_in0, void* out,
+int vx_mlp_pack_net(int prec, int n_hidden, const int64_t* w_ptrs, int k_in0, void* out,
                     void* stream) {
+  VX_CHECK_ARG(prec >= 0 && prec <= 2, "mlp_pack_net: prec must be 0, 1 or 2");
   VX_CHECK_ARG(n_hidden >= 1 && 2 * n_hidden <= kMaxPackSegs, "mlp_pack_net: 1..%d hidden layers",
                kMaxPackSegs / 2);
   VX_CHECK_ARG(k_in0 >= 1 && k_in0 <= 96, "mlp_pack_net: input width must be 1..96");
@@ -740,22 +1048,30 @@ int vx_mlp_pack_net(int bf16, int n_hidden, const int64_t* w_ptrs, int k_in0, vo
   }
   dim3 grid(grid_for(biggest, 256), 2 * n_hidden);
   hipStream_t st = (hipStream_t)stream;
-  if (bf16) k_pack_net<true><<<grid, 256, 0, st>>>(net, out);
-  else k_pack_net<false><<<grid, 256, 0, st>>>(net, out);
+  if (prec == 2) k_pack_net<2><<<grid, 256, 0, st>>>(net, out);
+  else if (prec == 1) k_pack_net<1><<<grid, 256, 0, st>>>(net, out);
+  else k_pack_net<0><<<grid, 256, 0, st>>>(net, out);
   return launch_status("mlp_pack_net");
 }
 
-int vx_mlp_layer_fwd(int bf16, int kt, int last, const void* in, int in_stride, int64_t M,
+int vx_mlp_layer_fwd(int prec, int kt, int last, const void* in, int in_stride, int64_t M,
                      const void* wpk, const float* bias, void* out, int out_stride,
                      const float* w3, const float* b3, float* logits, const int64_t* M_dev,
                      void* stream) {
+  VX_CHECK_ARG(prec >= 0 && prec <= 2, "mlp_layer_fwd: prec must be 0, 1 or 2");
+  const int bf16 = prec == 1;
   VX_CHECK_ARG(M >= 0, "mlp_layer_fwd: M < 0");
   VX_CHECK_ARG(kt == 2 || kt == 3 || kt == 6, "mlp_layer_fwd: input width must be 64, 96 or 192");
   if (M == 0) return VX_OK;
   VX_CHECK_ARG(!last || (w3 && b3 && logits), "mlp_layer_fwd: last layer needs w3, b3, logits");
   hipStream_t st = (hipStream_t)stream;
 #define L(KT, BF) launch_layer<KT, 6, 0, BF>(in, in_stride, M, M_dev, wpk, bias, nullptr, 0, out, out_stride, st)
-  if (!bf16) {
+#define LS(KT, NTL, NCG) launch_layer_split<KT, NTL, NCG, 0>(in, in_stride, M, M_dev, wpk, bias, nullptr, 0, out, out_stride, st)
+  if (prec == 2) {
+    if (kt == 2) LS(2, 3, 2);
+    else if (kt == 3) LS(3, 3, 2);
+    else LS(6, 3, 2);
+  } else if (!bf16) {
     if (kt == 2) L(2, false);
     else if (kt == 3) L(3, false);
     else L(6, false);
@@ -765,6 +1081,7 @@ int vx_mlp_layer_fwd(int bf16, int kt, int last, const void* in, int in_stride, 
     else L(6, true);
   }
 #undef L
+#undef LS
   if (last) {
     const unsigned g = grid_for(M, 4, 8 * (int64_t)num_cus());
     if (bf16) k_mlp_logits<true><<<g, 256, 0, st>>>(out, out_stride, M, M_dev, w3, b3, logits);
@@ -773,15 +1090,22 @@ int vx_mlp_layer_fwd(int bf16, int kt, int last, const void* in, int in_stride, 
   return launch_status("mlp_layer_fwd");
 }
 
-int vx_mlp_layer_bwd(int bf16, int nto, const float* dy, int dy_stride, int64_t M,
+int vx_mlp_layer_bwd(int prec, int nto, const float* dy, int dy_stride, int64_t M,
                      const void* wpk_t, const void* mask, int mask_stride, float* dx,
                      int dx_stride, const int64_t* M_dev, void* stream) {
+  VX_CHECK_ARG(prec >= 0 && prec <= 2, "mlp_layer_bwd: prec must be 0, 1 or 2");
+  const int bf16 = prec == 1;
   VX_CHECK_ARG(M >= 0, "mlp_layer_bwd: M < 0");
   VX_CHECK_ARG(nto == 2 || nto == 3 || nto == 6, "mlp_layer_bwd: output width must be 64, 96 or 192");
   if (M == 0) return VX_OK;
   hipStream_t st = (hipStream_t)stream;
 #define L(NTO, MODE, BF) launch_layer<6, NTO, MODE, BF>(dy, dy_stride, M, M_dev, wpk_t, nullptr, mask, mask_stride, dx, dx_stride, st)
-  if (!bf16) {
+#define LS(NTL, NCG, MODE) launch_layer_split<6, NTL, NCG, MODE>(dy, dy_stride, M, M_dev, wpk_t, nullptr, mask, mask_stride, dx, dx_stride, st)
+  if (prec == 2) {
+    if (nto == 6) { if (mask) LS(3, 2, 2); else LS(3, 2, 3); }
+    else if (nto == 3) LS(3, 1, 3);
+    else LS(1, 2, 3);
+  } else if (!bf16) {
     if (nto == 6) { if (mask) L(6, 2, false); else L(6, 3, false); }
     else if (nto == 3) L(3, 3, false);
     else L(2, 3, false);
@@ -791,6 +1115,7 @@ int vx_mlp_layer_bwd(int bf16, int nto, const float* dy, int dy_stride, int64_t 
     else L(2, 3, true);
   }
 #undef L
+#undef LS
   return launch_status("mlp_layer_bwd");
 }
 
@@ -811,9 +1136,11 @@ int64_t vx_mlp_wgrad_partb_elems(int64_t M) {
   return (nb + kReduceGroups) * kWidth;
 }
 
-int vx_mlp_wgrad(int bf16, int kt, int k_real, const float* dY, int dys, const void* X, int xs,
+int vx_mlp_wgrad(int prec, int kt, int k_real, const float* dY, int dys, const void* X, int xs,
                  int64_t M, float* part, float* partb, float* dW, float* db, const int64_t* M_dev,
                  void* stream) {
+  VX_CHECK_ARG(prec >= 0 && prec <= 2, "mlp_wgrad: prec must be 0, 1 or 2");
+  const int bf16 = prec == 1;
   VX_CHECK_ARG(kt == 2 || kt == 3 || kt == 6, "mlp_wgrad: input width must be 64, 96 or 192");
   VX_CHECK_ARG(k_real >= 1 && k_real <= 32 * kt, "mlp_wgrad: k_real out of range");
   VX_CHECK_ARG(M >= 0, "mlp_wgrad: M < 0");
@@ -831,9 +1158,11 @@ int vx_mlp_wgrad(int bf16, int kt, int k_real, const float* dY, int dys, const v
   return launch_status("mlp_wgrad");
 }
 
-int vx_mlp_last_bwd(int bf16, const float* dlogit, const void* H, int hs, int64_t M,
+int vx_mlp_last_bwd(int prec, const float* dlogit, const void* H, int hs, int64_t M,
                     const float* w3, float* dH, int dhs, float* part, float* partb, float* dW3,
                     float* db3, const int64_t* M_dev, void* stream) {
+  VX_CHECK_ARG(prec >= 0 && prec <= 2, "mlp_last_bwd: prec must be 0, 1 or 2");
+  const int bf16 = prec == 1;
   VX_CHECK_ARG(M >= 0, "mlp_last_bwd: M < 0");
   hipStream_t st = (hipStream_t)stream;
   const int nblk = last_blocks(M);

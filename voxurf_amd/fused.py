@@ -146,6 +146,11 @@ class FusedFineTrainer:
     _sgtv = None
 
     @property
+    def mlp_prec(self):
+        """Precision of the rgbnet kernels (voxurf_amd.mlp_hip.prec_of)."""
+        return mlp_hip.prec_of(self.rgbnet.compute_dtype, self.rgbnet.f32_split)
+
+    @property
     def k0(self):
         """The feature grid, once every pending update of it has finished (for reads on
         torch's current stream)."""
@@ -246,8 +251,7 @@ class FusedFineTrainer:
         self._gather(n, viewdirs)
         if k0_step is not None:
             lin = self.rgbnet.linears()
-            self._packed = mlp_hip.Packed([l.weight.detach() for l in lin],
-                                          self.rgbnet.compute_dtype == torch.bfloat16, st)
+            self._packed = mlp_hip.Packed([l.weight.detach() for l in lin], self.mlp_prec, st)
         if not sync:
             return None
         m = self._survivors(n)
@@ -302,7 +306,7 @@ class FusedFineTrainer:
         lin = self.rgbnet.linears()
         Ws = [l.weight.detach() for l in lin]
         bs = [l.bias.detach() for l in lin]
-        bf16 = self.rgbnet.compute_dtype == torch.bfloat16
+        prec = self.mlp_prec
         self._forward(rays_o, rays_d, viewdirs, pages, k0_step, sync=False)
         # The colour side is queued with the survivor count still on the device (off[n-1]):
         # every kernel reads it there and processes min(count, capacity) rows, so the GPU
@@ -313,8 +317,8 @@ class FusedFineTrainer:
         while True:
             rows = self._rows
             if n > 0:
-                Xr = mlp_hip.padded_rows(self._X, din, bf16)
-                lg, hs = mlp_hip.forward_rows(Xr, Ws, bs, bf16, self._packed, st, M_dev=m_dev)
+                Xr = mlp_hip.padded_rows(self._X, din, prec)
+                lg, hs = mlp_hip.forward_rows(Xr, Ws, bs, prec, self._packed, st, M_dev=m_dev)
             else:
                 lg = torch.zeros((1, 3), dtype=torch.float32, device=self.dev)
             dlogit = torch.empty((max(rows, 1), 3), dtype=torch.float32, device=self.dev)
@@ -323,7 +327,7 @@ class FusedFineTrainer:
                  ptr(self._P4), ptr(self.last), ptr(target), float(cfg.bg), ptr(self.rgb),
                  ptr(dlogit), ptr(gw), ptr(self.glast), ptr(self.S), ptr(self.loss_r), rows, st)
             if n > 0:
-                dX = mlp_hip.backward_rows(Xr, hs, Ws, bf16, self._packed, dlogit, st,
+                dX = mlp_hip.backward_rows(Xr, hs, Ws, prec, self._packed, dlogit, st,
                                            self.mlp_grads, M_dev=m_dev)
             else:
                 # no ray: every rgbnet gradient is zero

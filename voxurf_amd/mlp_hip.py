@@ -3,11 +3,20 @@
 mlp_forward(x, linears, compute_dtype) -> logits [M, 3], differentiable w.r.t. x and
 every Linear's weight and bias.  Shapes: width 192, at least one hidden layer of
 192 x 192, input width <= 96 (the plain fine step's rgbnet takes 42 inputs, the
-reference's fine rgbnet 79).  compute_dtype float32 runs FP32 MFMAs; bfloat16 runs the
-forward and data-gradient GEMMs as BF16 MFMAs with FP32 accumulation, stores the
-activations in BF16 and keeps weights, biases, gradients and the optimiser in FP32.
+reference's fine rgbnet 79).
 
-The pieces are also exposed without autograd (pack_net, forward_rows, backward_rows)
+Precision (`prec` of the kernels):
+  PREC_F32_SPLIT (float32, the default): float32 activations, weights and gradients;
+      every product is formed from three-way BF16 splits of both operands (six BF16
+      MFMA products per pair, FP32 accumulation), within 2^-23 relative of the exact
+      product, so the results carry FP32 accuracy (the weight gradients run on FP32
+      MFMAs);
+  PREC_F32: the same on FP32 MFMAs (v_mfma_f32_32x32x2f32);
+  PREC_BF16 (compute_dtype bfloat16): forward and data-gradient GEMMs as single BF16
+      MFMAs with FP32 accumulation, activations stored in BF16; weights, biases,
+      gradients and the optimiser stay FP32.
+
+The pieces are also exposed without autograd (Packed, forward_rows, backward_rows)
 for callers that keep the gradients in their own buffers (the fused training step).
 """
 import torch
@@ -15,6 +24,21 @@ import torch
 from ._lib import call, ptr, stream_of, lib
 
 WIDTH = 192
+PREC_F32, PREC_BF16, PREC_F32_SPLIT = 0, 1, 2
+
+
+def prec_of(compute_dtype, f32_split=True):
+    """Kernel precision for an rgbnet compute dtype (float32 or bfloat16)."""
+    if compute_dtype == torch.bfloat16:
+        return PREC_BF16
+    if compute_dtype == torch.float32:
+        return PREC_F32_SPLIT if f32_split else PREC_F32
+    raise RuntimeError("the rgbnet kernels compute in float32 or bfloat16")
+
+
+def act_dtype(prec):
+    """Element type of the stored activations."""
+    return torch.bfloat16 if prec == PREC_BF16 else torch.float32
 
 
 def _pad_width(d):
@@ -28,36 +52,39 @@ class Packed:
     """Every MFMA operand of one set of weights (one vx_mlp_pack_net launch):
     fwd[li] (layer li, forward) and bwd[li] (layer li transposed, backward)."""
 
-    def __init__(self, Ws, bf16, st):
+    def __init__(self, Ws, prec, st):
         n_hidden = len(Ws) - 1
         k_in0 = Ws[0].shape[1]
         total = lib().vx_mlp_pack_net_elems(n_hidden, k_in0)
         if total < 0:
             raise RuntimeError("rgbnet shape not supported by the MFMA kernels")
+        parts = 3 if prec == PREC_F32_SPLIT else 1
         dev = Ws[0].device
-        self.buf = torch.empty(total, dtype=torch.bfloat16 if bf16 else torch.float32, device=dev)
+        self.buf = torch.empty(parts * total, dtype=torch.float32 if prec == PREC_F32 else torch.bfloat16,
+                               device=dev)
         w_ptrs = torch.tensor([W.data_ptr() for W in Ws[:-1]], dtype=torch.int64)
-        call("vx_mlp_pack_net", int(bf16), n_hidden, w_ptrs.data_ptr(), k_in0, ptr(self.buf), st)
+        call("vx_mlp_pack_net", prec, n_hidden, w_ptrs.data_ptr(), k_in0, ptr(self.buf), st)
         kt0 = _pad_width(k_in0) // 32
         self.fwd, self.bwd = [], []
         off = 0
         for li in range(n_hidden):
             kt = kt0 if li == 0 else 6
-            n = lib().vx_mlp_pack_elems(kt, 6)
+            n = parts * lib().vx_mlp_pack_elems(kt, 6)
             self.fwd.append(self.buf[off:off + n])
             off += n
             self.bwd.append(self.buf[off:off + n])
             off += n
         self.kt0 = kt0
+        self.prec = prec
 
 
-def forward_rows(X, Ws, bs, bf16, packed, st, M_dev=None):
+def forward_rows(X, Ws, bs, prec, packed, st, M_dev=None):
     """logits [M, 3] and the hidden activations of X [M, 32*kt0] (padded rows in the
-    compute element type).  M_dev: optional device int64 row count (at most M rows of
+    activation element type).  M_dev: optional device int64 row count (at most M rows of
     the buffers are then used), so the launches need not wait for it on the host."""
     M = X.shape[0]
     dev = X.device
-    adt = torch.bfloat16 if bf16 else torch.float32
+    adt = act_dtype(prec)
     logits = torch.empty((M, 3), dtype=torch.float32, device=dev)
     hs = []
     h_in, kt = X, packed.kt0
@@ -65,7 +92,7 @@ def forward_rows(X, Ws, bs, bf16, packed, st, M_dev=None):
     for li in range(nlin - 1):
         last = li == nlin - 2
         H = torch.empty((M, WIDTH), dtype=adt, device=dev)
-        call("vx_mlp_layer_fwd", int(bf16), kt, int(last), ptr(h_in), h_in.shape[1], M,
+        call("vx_mlp_layer_fwd", prec, kt, int(last), ptr(h_in), h_in.shape[1], M,
              ptr(packed.fwd[li]), ptr(bs[li]), ptr(H), WIDTH, ptr(Ws[-1]) if last else None,
              ptr(bs[-1]) if last else None, ptr(logits) if last else None, ptr(M_dev), st)
         hs.append(H)
@@ -73,7 +100,7 @@ def forward_rows(X, Ws, bs, bf16, packed, st, M_dev=None):
     return logits, hs
 
 
-def backward_rows(X, hs, Ws, bf16, packed, g, st, grads, M_dev=None):
+def backward_rows(X, hs, Ws, prec, packed, g, st, grads, M_dev=None):
     """Gradients of every weight and bias into the tensors of `grads` ([W0, b0, W1, ...],
     overwritten) and the gradient of X [M, 32*kt0] (returned, float32); M_dev as for
     forward_rows."""
@@ -84,7 +111,7 @@ def backward_rows(X, hs, Ws, bf16, packed, g, st, grads, M_dev=None):
     partb = torch.empty(lib().vx_mlp_wgrad_partb_elems(M), dtype=torch.float32, device=dev)
     # output layer (K = 3) and the gradient of the last hidden activations
     dH = torch.empty((M, WIDTH), dtype=torch.float32, device=dev)
-    call("vx_mlp_last_bwd", int(bf16), ptr(g), ptr(hs[-1]), WIDTH, M, ptr(Ws[-1]), ptr(dH),
+    call("vx_mlp_last_bwd", prec, ptr(g), ptr(hs[-1]), WIDTH, M, ptr(Ws[-1]), ptr(dH),
          WIDTH, ptr(part), ptr(partb), ptr(grads[2 * (nlin - 1)]), ptr(grads[2 * nlin - 1]),
          ptr(M_dev), st)
     dX = None
@@ -92,27 +119,27 @@ def backward_rows(X, hs, Ws, bf16, packed, g, st, grads, M_dev=None):
         h_prev = hs[li - 1] if li > 0 else X
         kt = 6 if li > 0 else packed.kt0
         # weight gradient of layer li: dY = dH (its output grad), X = its input
-        call("vx_mlp_wgrad", int(bf16), kt, Ws[li].shape[1], ptr(dH), WIDTH, ptr(h_prev),
+        call("vx_mlp_wgrad", prec, kt, Ws[li].shape[1], ptr(dH), WIDTH, ptr(h_prev),
              h_prev.shape[1], M, ptr(part), ptr(partb), ptr(grads[2 * li]),
              ptr(grads[2 * li + 1]), ptr(M_dev), st)
         # data gradient through layer li
         if li > 0:
             dprev = torch.empty((M, WIDTH), dtype=torch.float32, device=dev)
-            call("vx_mlp_layer_bwd", int(bf16), kt, ptr(dH), WIDTH, M, ptr(packed.bwd[li]),
+            call("vx_mlp_layer_bwd", prec, kt, ptr(dH), WIDTH, M, ptr(packed.bwd[li]),
                  ptr(h_prev), WIDTH, ptr(dprev), WIDTH, ptr(M_dev), st)
             dH = dprev
         else:
             dX = torch.empty((M, 32 * kt), dtype=torch.float32, device=dev)
-            call("vx_mlp_layer_bwd", int(bf16), kt, ptr(dH), WIDTH, M, ptr(packed.bwd[0]), None,
+            call("vx_mlp_layer_bwd", prec, kt, ptr(dH), WIDTH, M, ptr(packed.bwd[0]), None,
                  0, ptr(dX), 32 * kt, ptr(M_dev), st)
     return dX
 
 
-def padded_rows(x, k_in, bf16):
-    """x [M, d] -> contiguous [M, 32*kt0] rows in the compute element type (x is used in
-    place when it already has that width; columns past k_in meet zero weights)."""
+def padded_rows(x, k_in, prec):
+    """x [M, d] -> contiguous [M, 32*kt0] rows in the activation element type (x is used
+    in place when it already has that width; columns past k_in meet zero weights)."""
     kin = _pad_width(k_in)
-    adt = torch.bfloat16 if bf16 else torch.float32
+    adt = act_dtype(prec)
     if x.shape[1] == kin and x.is_contiguous():
         return x.detach().to(adt)
     return torch.nn.functional.pad(x.detach(), (0, kin - x.shape[1])).to(adt).contiguous()
@@ -120,17 +147,17 @@ def padded_rows(x, k_in, bf16):
 
 class _MlpFn(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, x, bf16, *params):
+    def forward(ctx, x, prec, *params):
         Ws = [p.detach().contiguous() for p in params[0::2]]
         bs = [p.detach().contiguous() for p in params[1::2]]
         if any(w.dtype != torch.float32 for w in Ws):
             raise RuntimeError("rgbnet parameters must be float32")
         st = stream_of(x)
-        X = padded_rows(x, Ws[0].shape[1], bf16)
-        packed = Packed(Ws, bf16, st)
-        logits, hs = forward_rows(X, Ws, bs, bf16, packed, st)
+        X = padded_rows(x, Ws[0].shape[1], prec)
+        packed = Packed(Ws, prec, st)
+        logits, hs = forward_rows(X, Ws, bs, prec, packed, st)
         ctx.save_for_backward(X, *hs)
-        ctx.Ws, ctx.bs, ctx.packed, ctx.bf16, ctx.din = Ws, bs, packed, bf16, x.shape[1]
+        ctx.Ws, ctx.bs, ctx.packed, ctx.prec, ctx.din = Ws, bs, packed, prec, x.shape[1]
         return logits
 
     @staticmethod
@@ -141,7 +168,7 @@ class _MlpFn(torch.autograd.Function):
         grads = []
         for W, b in zip(Ws, bs):
             grads += [torch.empty_like(W), torch.empty_like(b)]
-        dX = backward_rows(X, hs, Ws, ctx.bf16, ctx.packed, glogits.contiguous().float(),
+        dX = backward_rows(X, hs, Ws, ctx.prec, ctx.packed, glogits.contiguous().float(),
                            stream_of(X), grads)
         return (dX[:, :ctx.din], None, *grads)
 
@@ -152,8 +179,9 @@ def check_shapes(linears):
         raise RuntimeError("rgbnet kernels support width 192, depth >= 3, 3 outputs")
 
 
-def mlp_forward(x, linears, compute_dtype=torch.float32, padded=False):
-    """logits = rgbnet(x) on the MFMA kernels (float32 or bfloat16 compute).
+def mlp_forward(x, linears, compute_dtype=torch.float32, padded=False, f32_split=True):
+    """logits = rgbnet(x) on the MFMA kernels (float32 or bfloat16 compute; float32 by
+    BF16 splits unless f32_split is False).
 
     padded=True: x [M, W] float32 rows already padded to the kernels' tile width W
     (64 or 96, the smallest >= in_features) with finite values past in_features; they
@@ -161,7 +189,7 @@ def mlp_forward(x, linears, compute_dtype=torch.float32, padded=False):
     if not x.is_cuda:
         raise RuntimeError("the rgbnet kernels need CUDA (HIP) tensors")
     check_shapes(linears)
-    bf16 = compute_dtype == torch.bfloat16
+    prec = prec_of(compute_dtype, f32_split)
     params = []
     for l in linears:
         params += [l.weight, l.bias]
@@ -174,5 +202,5 @@ def mlp_forward(x, linears, compute_dtype=torch.float32, padded=False):
         if x.shape[-1] != linears[0].in_features:
             raise RuntimeError(f"rgbnet input width {x.shape[-1]} != {linears[0].in_features}")
         x2 = x.reshape(-1, x.shape[-1]).float()
-    out = _MlpFn.apply(x2, bf16, *params)
+    out = _MlpFn.apply(x2, prec, *params)
     return out.reshape(*x.shape[:-1], 3)
