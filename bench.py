@@ -173,12 +173,12 @@ def main():
         adam_ms = sum(a.elapsed_time(b) for a, b in evs) / len(evs)
         out = run(args.warmup + args.steps, profile=True)
         torch.cuda.synchronize()
-        live = trainer.last_live_pages * 1024
+        live = trainer.last_live_pages * trainer.page
         if trainer.overlap_adam:
             # phase 1: the touched pages no survivor reaches this step -- param,
             # exp_avg, exp_avg_sq read and written, the gradient known zero
             roof_kernel = "adam_step phase 1 (touched k0 pages with zero gradient; 256^3 x 12 fp32 grid)"
-            adam_bytes = 24 * trainer.last_swept_pages * 1024
+            adam_bytes = 24 * trainer.last_swept_pages * trainer.page
         else:
             roof_kernel = "adam_step (k0 grid, 256^3 x 12 fp32, page-flagged gradient)"
             adam_bytes = 24 * n_k0 + 8 * live

@@ -286,28 +286,32 @@ int vx_adam_upd(int dtype, void* param, const void* grad, void* exp_avg, void* e
 /* lib/utils.py:154-199 adam() for one parameter (the optimiser the training step
  * runs): step is the 1-based step count after increment; per_lr may be NULL;
  * clear_grad != 0 zeroes grad after use.  pages (may be NULL; float32 with clear_grad
- * only): one flag per 1024-element page of grad, nonzero where anything was added to
- * that page since the last step; pages flagged zero are all-zero gradient and are
- * neither read nor cleared; the flags are reset to zero. */
+ * only): one flag per page of grad (vx_adam_page_elems() elements), nonzero where
+ * anything was added to that page since the last step; pages flagged zero are all-zero
+ * gradient and are neither read nor cleared; the flags are reset to zero. */
 int vx_adam_step(int dtype, void* param, void* grad, void* exp_avg, void* exp_avg_sq,
                  const void* per_lr, int64_t n, int step, double lr, double beta1, double beta2,
                  double eps, int clear_grad, uint32_t* pages, void* stream);
+/* Elements per gradient page (the page flags of vx_adam_step, vx_adam_step_split and
+ * the fused forward; the touched words of vx_adam_step_touched): 256.  No counterpart in
+ * the reference, which updates every element. */
+int64_t vx_adam_page_elems(void);
 /* lib/utils.py:169-199 utils.Adam on one float32 parameter (clear_grad = 1, 16-byte
- * aligned buffers), skipping pages of 1024 elements whose update is the identity:
- * touched word zero (moments still zero) and an all-zero gradient this step.  A page with
- * a nonzero gradient is marked touched.  touched: ceil(n / 1024) words, zero-initialised
- * with the moments (all ones after loading moments from elsewhere). */
+ * aligned buffers), skipping pages whose update is the identity: touched word zero
+ * (moments still zero) and an all-zero gradient this step.  A page with a nonzero
+ * gradient is marked touched.  touched: ceil(n / vx_adam_page_elems()) words,
+ * zero-initialised with the moments (all ones after loading moments from elsewhere). */
 int vx_adam_step_touched(float* param, float* grad, float* exp_avg, float* exp_avg_sq,
                          const float* per_lr, int64_t n, int step, double lr, double beta1,
                          double beta2, double eps, uint32_t* touched, void* stream);
 /* vx_adam_step (float32, clear_grad = 1, 16-byte aligned buffers) split in two passes
- * over disjoint element sets, for overlap with other work.  pages holds
- * 2 * ceil(n / 1024) + 1 words: the page flags, a count and a page list (count and list
- * zero-initialised once).  Phase 1 updates the pages flagged zero (all-zero gradient, so
+ * over disjoint element sets, for overlap with other work.  pages holds 2 * P + 1 words
+ * (P = ceil(n / vx_adam_page_elems())): the page flags, a count and a page list (count
+ * and list zero-initialised once).  Phase 1 updates the pages flagged zero (all-zero gradient, so
  * it may run as soon as the flags are final, before the gradient is) from one block per
  * CU, and lists the flagged pages; phase 2 updates the listed pages, clears their
  * gradient and resets flags and count.  Phase 1 then phase 2 with the same step equal
- * one vx_adam_step.  touched (ceil(n / 1024) words, or NULL): nonzero for every page whose
+ * one vx_adam_step.  touched (one word per page, or NULL): nonzero for every page whose
  * moments may be nonzero; phase 1 skips pages with a zero flag and a zero touched word
  * (their p, m and v are left exactly as utils.Adam leaves them: m = v = g = 0 makes the
  * update an identity) and phase 2 marks the pages it updates.  Zero-initialise it with

@@ -6,15 +6,17 @@ tail of n % 4 elements."""
 import pytest
 import torch
 
-from voxurf_amd._lib import call, ptr
+from voxurf_amd._lib import call, lib, ptr
 
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("n", [1024 * 37, 1024 * 37 + 500, 1024 * 5 + 6])
-def test_adam_step_touched_equals_dense(dev, n):
+@pytest.mark.parametrize("pages,extra", [(37, 0), (37, 125), (5, 6), (300, 2)])
+def test_adam_step_touched_equals_dense(dev, pages, extra):
+    P = int(lib().vx_adam_page_elems())
+    n = P * pages + extra
     g = torch.Generator().manual_seed(n)
-    npg = (n + 1023) // 1024
+    npg = (n + P - 1) // P
     p = torch.randn(n, generator=g)
     m = torch.randn(n, generator=g) * 0.1
     v = torch.rand(n, generator=g) * 0.01
@@ -22,7 +24,7 @@ def test_adam_step_touched_equals_dense(dev, n):
     touched = (torch.rand(npg, generator=g) < 0.4).int()
     zero_grad = torch.rand(npg, generator=g) < 0.6
     for k in range(npg):
-        sl = slice(1024 * k, min(n, 1024 * (k + 1)))
+        sl = slice(P * k, min(n, P * (k + 1)))
         if not touched[k]:
             m[sl] = 0
             v[sl] = 0
@@ -43,9 +45,9 @@ def test_adam_step_touched_equals_dense(dev, n):
         want = (touched.bool() | ~zero_grad).int()
         assert torch.equal(tch.cpu(), want)
         # next step: fresh gradients on a different set of pages
-        gr2 = torch.where((torch.rand(npg, generator=g) < 0.3).repeat_interleave(1024)[:n],
+        gr2 = torch.where((torch.rand(npg, generator=g) < 0.3).repeat_interleave(P)[:n],
                           torch.randn(n, generator=g), torch.zeros(n))
         a[1].copy_(gr2)
         b[1].copy_(gr2)
         touched = tch.cpu().clone()
-        zero_grad = torch.tensor([not bool(gr2[1024 * k:1024 * (k + 1)].any()) for k in range(npg)])
+        zero_grad = torch.tensor([not bool(gr2[P * k:P * (k + 1)].any()) for k in range(npg)])

@@ -187,6 +187,6 @@ def test_touched_page_skip_over_changing_batches(dev, setup):
     flat = lambda x: x.detach().permute(0, 2, 3, 4, 1).reshape(-1).cpu()  # noqa: E731
     k0_init = st["k0"].permute(0, 2, 3, 4, 1).reshape(-1)
     n = k0_init.numel()
-    sel = untouched.repeat_interleave(1024)[:n]
+    sel = untouched.repeat_interleave(t.page)[:n]
     assert torch.equal(flat(t.k0)[sel], k0_init[sel])
     assert not bool(flat(t.adam["k0"][0])[sel].any()) and not bool(flat(t.adam["k0"][1])[sel].any())

@@ -8,6 +8,11 @@
 
 namespace vx {
 
+// Gradient pages: the granularity of the page flags the fused forward sets for the
+// feature scatter and of the page-skipping Adam passes (256 elements, 1 KiB of float32).
+constexpr int kPageShift = 8;
+constexpr int kPageElems = 1 << kPageShift;
+
 // Thread-local text of the last failure; returned by vx_last_error().
 void set_error(const char* fmt, ...);
 

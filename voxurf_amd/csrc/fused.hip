@@ -73,7 +73,7 @@ __device__ __forceinline__ void run_chunk(const FusedGeom& G, const float* start
 }
 
 __device__ __forceinline__ void mark_page(uint32_t* pages, int64_t addr) {
-  pages[addr >> 10] = 1u;
+  pages[addr >> kPageShift] = 1u;
 }
 
 // Flag the page(s) holding the C channels of trilinear corner `corner` (0..7, x outer,

@@ -48,9 +48,13 @@ __device__ __forceinline__ void adam_elem(float& p, float g, float& m, float& v,
 }
 
 // Streaming pass over 16-byte vectors.  `pages` (optional) holds one flag per
-// kPage-element page of the gradient: a zero flag means the whole page of the gradient
-// is zero (nothing was scattered into it this step), so it is neither read nor cleared.
-constexpr int kPageShift = 10;  // 1024 elements
+// kPageElems-element page of the gradient: a zero flag means the whole page of the
+// gradient is zero (nothing was scattered into it this step), so it is neither read nor
+// cleared.  The page passes below give each page to kPageVec consecutive threads (one
+// float4 each), kPagesPerRow pages per 256-thread block row.
+constexpr int kPageVec = kPageElems / 4;
+constexpr int kPagesPerRow = 256 / kPageVec;
+static_assert(kPageVec == 64, "a page is one wavefront of float4");
 
 typedef float nt_f4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ float4 ldnt(const float4* p) {
@@ -127,12 +131,12 @@ __global__ void __launch_bounds__(256) k_adam_step_f32(float* __restrict__ p, fl
 // The two halves of vx_adam_step_split.  `pages` holds [flags (npages) | count | list
 // (npages)]: phase 1 updates the pages whose flag is zero (and that were touched) and
 // appends every flagged page to the list; phase 2 updates the listed pages.  A page is
-// 1024 elements = 256 float4, one per thread of a 256-thread block.
+// one wavefront's float4 (kPageVec), kPagesPerRow pages per block row.
 //
 // Phase 1 takes the pages 256 at a time: each thread classifies one page (one load of
 // its flag and touched word), the block compacts the pages to sweep into LDS and appends
-// the flagged ones to the list with one atomic, then sweeps its pages U at a time with
-// every load issued before the first store.
+// the flagged ones to the list with one atomic, then sweeps its pages U rows at a time
+// with every load issued before the first store.
 template <int U>
 __global__ void __launch_bounds__(256) k_adam_dead_pages(float* __restrict__ p, float* __restrict__ m,
                                                          float* __restrict__ v,
@@ -179,12 +183,13 @@ __global__ void __launch_bounds__(256) k_adam_dead_pages(float* __restrict__ p, 
     __syncthreads();
     if (live) list[live_base + lo + (uint32_t)__popcll(lb & below)] = (uint32_t)pg;
     // 2. sweep this chunk's pages, U at a time
-    for (uint32_t k0 = 0; k0 < ndead; k0 += U) {
+    for (uint32_t k0 = 0; k0 < ndead; k0 += U * kPagesPerRow) {
       float4 pp[U], mm[U], vv[U];
       int64_t idx[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        idx[u] = k0 + u < ndead ? (int64_t)sweep[k0 + u] * 256 + tid : -1;
+        const uint32_t slot = k0 + u * kPagesPerRow + tid / kPageVec;
+        idx[u] = slot < ndead ? (int64_t)sweep[slot] * kPageVec + tid % kPageVec : -1;
         if (idx[u] >= 0 && idx[u] < n4) {
           pp[u] = ldnt(p4 + idx[u]);
           mm[u] = ldnt(m4 + idx[u]);
@@ -234,17 +239,20 @@ __global__ void __launch_bounds__(256) k_adam_live_pages(float* __restrict__ p, 
   float4* v4 = reinterpret_cast<float4*>(v);
   const float4* l4 = reinterpret_cast<const float4*>(perlr);
   const int tid = threadIdx.x;
-  // four listed pages per block iteration: every load is issued before the first store
-  constexpr int U = 4;
-  for (int64_t k0 = (int64_t)blockIdx.x * U; k0 < (int64_t)cnt; k0 += (int64_t)gridDim.x * U) {
+  // U rows of listed pages per block iteration: every load is issued before the first
+  // store
+  constexpr int U = 4, PB = U * kPagesPerRow;
+  for (int64_t k0 = (int64_t)blockIdx.x * PB; k0 < (int64_t)cnt; k0 += (int64_t)gridDim.x * PB) {
     float4 pp[U], gg[U], mm[U], vv[U];
     int64_t idx[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       idx[u] = -1;
-      if (k0 + u < (int64_t)cnt) {
-        idx[u] = (int64_t)list[k0 + u] * 256 + tid;
-        if (touched && tid == 0) touched[list[k0 + u]] = 1u;
+      const int64_t slot = k0 + u * kPagesPerRow + tid / kPageVec;
+      if (slot < (int64_t)cnt) {
+        const uint32_t pg = list[slot];
+        idx[u] = (int64_t)pg * kPageVec + tid % kPageVec;
+        if (touched && tid % kPageVec == 0) touched[pg] = 1u;
       }
       if (idx[u] >= 0 && idx[u] < n4) {
         pp[u] = ldnt(p4 + idx[u]);
@@ -305,9 +313,9 @@ __global__ void k_adam_step_f64(double* __restrict__ p, double* __restrict__ g,
 
 // Dense utils.Adam pass that skips pages whose update is the identity: a page never
 // touched before (touched word zero: its moments are still zero) with an all-zero
-// gradient this step.  Each block takes U pages per iteration; the gradient of a page
-// is read first, the page's parameters and moments only if it is touched or its
-// gradient is nonzero (which marks it touched from now on).
+// gradient this step.  Each block takes U pages per iteration; a touched page's
+// gradient, parameters and moments are read together, an untouched page's parameters
+// and moments only if its gradient is nonzero (which marks it touched from now on).
 template <int U>
 __global__ void __launch_bounds__(256) k_adam_touched(float* __restrict__ p, float* __restrict__ g,
                                                       float* __restrict__ m, float* __restrict__ v,
@@ -321,30 +329,44 @@ __global__ void __launch_bounds__(256) k_adam_touched(float* __restrict__ p, flo
   float4* v4 = reinterpret_cast<float4*>(v);
   const float4* l4 = reinterpret_cast<const float4*>(perlr);
   const int tid = threadIdx.x;
-  for (int64_t pg0 = (int64_t)blockIdx.x * U; pg0 < npages; pg0 += (int64_t)gridDim.x * U) {
-    float4 gg[U];
+  constexpr int PB = U * kPagesPerRow;
+  for (int64_t pg0 = (int64_t)blockIdx.x * PB; pg0 < npages; pg0 += (int64_t)gridDim.x * PB) {
+    float4 gg[U], pp[U], mm[U], vv[U];
+    int64_t pg[U];
     bool was[U], nz[U];
+    // Touched pages: gradient and state loads issued together (one memory round trip);
+    // an untouched page needs its state only if its gradient turns out nonzero (a page
+    // touched for the first time), which the second round below loads.  A page is one
+    // wavefront, so "any nonzero in the page" is a wave vote.
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int64_t i = (pg0 + u) * 256 + tid;
-      was[u] = pg0 + u < npages && touched[pg0 + u] != 0u;
+      pg[u] = pg0 + u * kPagesPerRow + tid / kPageVec;
+      was[u] = pg[u] < npages && touched[pg[u]] != 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t i = pg[u] * kPageVec + tid % kPageVec;
       nz[u] = false;
       gg[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (pg0 + u < npages && i < n4) {
+      if (pg[u] < npages && i < n4) {
         gg[u] = ldnt(g4 + i);
+        if (was[u]) {
+          pp[u] = ldnt(p4 + i);
+          mm[u] = ldnt(m4 + i);
+          vv[u] = ldnt(v4 + i);
+        }
         nz[u] = gg[u].x != 0.f || gg[u].y != 0.f || gg[u].z != 0.f || gg[u].w != 0.f;
-      } else if (pg0 + u < npages && i == n4) {
+      } else if (pg[u] < npages && i == n4) {
         for (int64_t e = n4 << 2; e < n; ++e) nz[u] = nz[u] || g[e] != 0.f;
       }
     }
     bool any[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) any[u] = __syncthreads_or(nz[u]) != 0;
-    float4 pp[U], mm[U], vv[U];
+    for (int u = 0; u < U; ++u) any[u] = __any(nz[u]);
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int64_t i = (pg0 + u) * 256 + tid;
-      if ((was[u] || any[u]) && pg0 + u < npages && i < n4) {
+      const int64_t i = pg[u] * kPageVec + tid % kPageVec;
+      if (!was[u] && any[u] && pg[u] < npages && i < n4) {
         pp[u] = ldnt(p4 + i);
         mm[u] = ldnt(m4 + i);
         vv[u] = ldnt(v4 + i);
@@ -352,8 +374,8 @@ __global__ void __launch_bounds__(256) k_adam_touched(float* __restrict__ p, flo
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      if (!(was[u] || any[u]) || pg0 + u >= npages) continue;
-      const int64_t i = (pg0 + u) * 256 + tid;
+      if (!(was[u] || any[u]) || pg[u] >= npages) continue;
+      const int64_t i = pg[u] * kPageVec + tid % kPageVec;
       if (i < n4) {
         const float4 ll = perlr ? l4[i] : make_float4(1.f, 1.f, 1.f, 1.f);
         adam_elem(pp[u].x, gg[u].x, mm[u].x, vv[u].x, ll.x, c);
@@ -374,7 +396,7 @@ __global__ void __launch_bounds__(256) k_adam_touched(float* __restrict__ p, flo
           g[e] = 0.f;
         }
       }
-      if (any[u] && !was[u] && tid == 0) touched[pg0 + u] = 1u;
+      if (any[u] && !was[u] && tid % kPageVec == 0) touched[pg[u]] = 1u;
     }
   }
 }
@@ -454,6 +476,8 @@ int vx_adam_step(int dtype, void* param, void* grad, void* exp_avg, void* exp_av
   return launch_status("adam_step");
 }
 
+int64_t vx_adam_page_elems(void) { return kPageElems; }
+
 int vx_adam_step_touched(float* param, float* grad, float* exp_avg, float* exp_avg_sq,
                          const float* per_lr, int64_t n, int step, double lr, double beta1,
                          double beta2, double eps, uint32_t* touched, void* stream) {
@@ -480,7 +504,9 @@ int vx_adam_step_touched(float* param, float* grad, float* exp_avg, float* exp_a
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
     cus = 256;
   constexpr int U = 4;
-  const int64_t blocks = std::min<int64_t>(8 * (int64_t)cus, cdiv(np, U));
+  // one block iteration per block up to 16 blocks per CU (each iteration is two
+  // dependent memory round trips: the page flags, then the data)
+  const int64_t blocks = std::min<int64_t>(16 * (int64_t)cus, cdiv(np, U * kPagesPerRow));
   k_adam_touched<U><<<(unsigned)blocks, 256, 0, (hipStream_t)stream>>>(
       param, grad, exp_avg, exp_avg_sq, per_lr, n, c, touched, np);
   return launch_status("adam_step_touched");

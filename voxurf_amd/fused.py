@@ -67,12 +67,13 @@ class FusedFineTrainer:
         self._k0_done = None         # event: the last step's k0 update has finished
         self.sdf_grad = torch.zeros_like(self.sdf)
         self.k0_grad = torch.zeros_like(self._k0)
-        # Per step, one flag per 1024-element page of k0_grad, set by the forward for every
-        # page the feature scatter can write, followed by the count and list of flagged
-        # pages the split Adam builds (vx_adam_step_split).  Two sets, alternating by
-        # step, because a step's k0 update may still be running when the next forward
+        # Per step, one flag per page of k0_grad (PAGE elements), set by the forward for
+        # every page the feature scatter can write, followed by the count and list of
+        # flagged pages the split Adam builds (vx_adam_step_split).  Two sets, alternating
+        # by step, because a step's k0 update may still be running when the next forward
         # flags its pages.
-        n_pages = (self._k0.numel() + 1023) // 1024
+        self.page = page = int(_lib.lib().vx_adam_page_elems())
+        n_pages = (self._k0.numel() + page - 1) // page
         self._pages = [torch.zeros(2 * n_pages + 1, dtype=torch.int32, device=self.dev)
                        for _ in range(2)]
         self._parity = 0
@@ -88,7 +89,7 @@ class FusedFineTrainer:
         self._touched = torch.zeros(n_pages, dtype=torch.int32, device=self.dev)
         # the same for the SDF grid, whose one-pass update skips untouched pages with an
         # all-zero gradient (vx_adam_step_touched)
-        self._sdf_touched = torch.zeros((self.sdf.numel() + 1023) // 1024, dtype=torch.int32,
+        self._sdf_touched = torch.zeros((self.sdf.numel() + page - 1) // page, dtype=torch.int32,
                                         device=self.dev)
         self.adam = {
             "sdf": [torch.zeros_like(self.sdf), torch.zeros_like(self.sdf), 0, cfg.lr_sdf],
@@ -353,8 +354,15 @@ class FusedFineTrainer:
             # pages phase 1 swept this step: touched before it and not flagged (phase 2
             # of this step has not run yet, so touched is still the state phase 1 read)
             self.last_swept_pages = int(((self._touched != 0) & ~flags).sum())
+        call("vx_fused_fine_bwd", ptr(self.sdf), self.sdf_geom.data_ptr(), self.box.data_ptr(),
+             self.params.data_ptr(), ptr(viewdirs), n, self.cap, ptr(self.s4), ptr(self.g4),
+             ptr(self.stats), ptr(self.cnt), ptr(self.off), ptr(self.last), ptr(gw),
+             ptr(self.glast), ptr(self.S), ptr(dgf), ptr(self.sdf_grad), rows, st)
         if self.overlap_adam:
-            # the flagged pages, once the feature scatter has finished
+            # The flagged pages, from the end of the SDF backward: the sweep (HBM-bound)
+            # then runs beside the SDF Adam pass and the next step's forward, which does
+            # not read k0 (the next gather waits for it), instead of beside the SDF
+            # backward's scattered atomics.
             grads_ready = torch.cuda.Event()
             grads_ready.record(main)
             with torch.cuda.stream(self.side):
@@ -368,10 +376,6 @@ class FusedFineTrainer:
             # the one-pass update does not record touched pages: count every page as
             # touched from now on (skipping is then off, which is always exact)
             self._touched.fill_(1)
-        call("vx_fused_fine_bwd", ptr(self.sdf), self.sdf_geom.data_ptr(), self.box.data_ptr(),
-             self.params.data_ptr(), ptr(viewdirs), n, self.cap, ptr(self.s4), ptr(self.g4),
-             ptr(self.stats), ptr(self.cnt), ptr(self.off), ptr(self.last), ptr(gw),
-             ptr(self.glast), ptr(self.S), ptr(dgf), ptr(self.sdf_grad), rows, st)
         tv_loss = self._regularise(n, k0_step) if tv_active(self.tv, k0_step) else None
         if profile:
             self.grads_snapshot = dict(sdf=self.sdf_grad.clone(), k0=k0_grad_snapshot,
@@ -445,7 +449,7 @@ class FusedFineTrainer:
             cells = (idx[:, None, :] + off[None]).reshape(-1, 3)
             cells = torch.minimum(torch.maximum(cells, torch.zeros_like(R)), R - 1)
             lin = (cells[:, 0] * R[1] + cells[:, 1]) * R[2] + cells[:, 2]
-            self._sdf_touched[torch.unique(lin // 1024)] = 1
+            self._sdf_touched[torch.unique(lin // self.page)] = 1
         self._touched |= (flags[:self.n_pages] != 0).to(torch.int32)
 
     def touched_pages(self):

@@ -61,6 +61,7 @@ def kernels(args):
     """Per-kernel rooflines: the Adam sweep over a 256^3 x 12 grid (with and without
     page flags), a plain 1 GiB device copy (the calibration kernel for HBM counters),
     and rgbnet forward+backward at the fine step's survivor count (FP32 and BF16)."""
+    from . import _lib
     from ._lib import call, ptr, stream_of
     from .mlp import RgbNet
     dev = torch.device("cuda:0")
@@ -78,7 +79,8 @@ def kernels(args):
 
     ms = _event_ms(adam)
     res["adam_dense"] = {"ms": ms, "bytes": 32 * n, "GBs": 32 * n / ms / 1e6}
-    pages = torch.zeros((n + 1023) // 1024, dtype=torch.int32, device=dev)
+    P = int(_lib.lib().vx_adam_page_elems())
+    pages = torch.zeros((n + P - 1) // P, dtype=torch.int32, device=dev)
     frac = 0.25
 
     def adam_paged():
