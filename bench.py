@@ -22,6 +22,7 @@ sys.path.insert(0, ROOT)
 
 # Peak figures of an MI355X used for roofline fractions.
 PEAK_HBM_GBS = 8000.0
+PEAK_FP32_MFMA_TF = 157.3
 
 
 def parse():
@@ -147,6 +148,7 @@ def main():
     torch.cuda.synchronize()
     if args.path == "fused":
         trainer.adam_events = []
+        trainer.layer_events = []
     stream = torch.cuda.current_stream()
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
     t0 = time.perf_counter()
@@ -164,38 +166,58 @@ def main():
     per.sort()
     ms = sum(per) / len(per)
 
-    # Roofline of the dominant kernel: the k0 Adam sweep (HBM-bound).  Its launches are
-    # timed with HIP events on the stream they run on, over the timed steps.
+    # The k0 Adam sweep (HBM-bound): its launches are timed with HIP events on the stream
+    # they run on, over the timed steps.
     n_k0 = cfg.R ** 3 * cfg.feat_dim
+    roofline = None
     if args.path == "fused":
         evs = trainer.adam_events
         trainer.adam_events = None
         adam_ms = sum(a.elapsed_time(b) for a, b in evs) / len(evs)
+        lev = trainer.layer_events
+        trainer.layer_events = None
         out = run(args.warmup + args.steps, profile=True)
         torch.cuda.synchronize()
         live = trainer.last_live_pages * trainer.page
         if trainer.overlap_adam:
             # phase 1: the touched pages no survivor reaches this step -- param,
             # exp_avg, exp_avg_sq read and written, the gradient known zero
-            roof_kernel = "adam_step phase 1 (touched k0 pages with zero gradient; 256^3 x 12 fp32 grid)"
+            adam_kernel = "adam_step phase 1 (touched k0 pages with zero gradient; 256^3 x 12 fp32 grid)"
             adam_bytes = 24 * trainer.last_swept_pages * trainer.page
         else:
-            roof_kernel = "adam_step (k0 grid, 256^3 x 12 fp32, page-flagged gradient)"
+            adam_kernel = "adam_step (k0 grid, 256^3 x 12 fp32, page-flagged gradient)"
             adam_bytes = 24 * n_k0 + 8 * live
+        # Roofline of the dominant kernel: the rgbnet 192 x 192 hidden layer forward (the
+        # layer kernels are the largest share of the step), timed with HIP events around
+        # each launch on its stream over the timed steps; algorithmic FLOPs 2 * 192 * 192
+        # per survivor row, against the FP32 matrix peak (the path computes in FP32; the
+        # split kernel executes six BF16 products per FP32 product, so its BF16 MFMA
+        # utilisation is also reported).
+        lay_ms = sum(a.elapsed_time(b) for a, b, _ in lev) / len(lev)
+        lay_flops = sum(2.0 * 192 * 192 * m for _, _, m in lev) / len(lev)
+        lay_tf = lay_flops / (lay_ms * 1e-3) / 1e12
+        prec = trainer.mlp_prec
+        peak = 2500.0 if prec == 1 else PEAK_FP32_MFMA_TF
+        traffic = None
+        tpath = os.path.join(ROOT, "profiles", "rgbnet_layer_traffic.json")
+        if prec == 2 and os.path.exists(tpath):
+            with open(tpath) as f:
+                traffic = json.load(f)["hbm_bytes_per_launch"]
+        roofline = {"bound": "mfma",
+                    "kernel": {0: "k_mlp_layer<6,6,0> (FP32 MFMA)", 1: "k_mlp_layer<6,6,0> (BF16 MFMA)",
+                               2: "k_mlp_layer_split<6,3,2,0> (FP32 via three-way BF16 splits)"}[prec]
+                    + ", rgbnet 192x192 hidden layer forward over the step's survivor rows",
+                    "ms": lay_ms, "algorithmic_flops": lay_flops, "achieved": lay_tf,
+                    "peak": peak, "unit": "TFLOP/s", "frac": lay_tf / peak, "traffic": traffic}
+        if prec == 2:
+            roofline["bf16_mfma_frac"] = 6 * lay_tf / 2500.0
     else:
         out = run(args.warmup + args.steps, profile=True)
         torch.cuda.synchronize()
         adam_ms = trainer.last_profile["adam_k0_ms"]
-        roof_kernel = "adam_step (k0 grid, 256^3 x 12 fp32)"
+        adam_kernel = "adam_step (k0 grid, 256^3 x 12 fp32)"
         adam_bytes = 32 * n_k0
-    achieved = adam_bytes / (adam_ms * 1e-3) / 1e9
-    # HBM bytes per launch of that kernel from rocprofv3 FETCH_SIZE / WRITE_SIZE passes
-    # (calibrated on a 1 GiB copy), when a measurement for it is on file
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "adam_phase1_traffic.json")
-    if args.path == "fused" and trainer.overlap_adam and os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f)["hbm_bytes_per_launch"]
+    adam_gbs = adam_bytes / (adam_ms * 1e-3) / 1e9
     counts = trainer.last_counts
     sb = step_bytes(cfg, counts)
 
@@ -217,10 +239,10 @@ def main():
         "config": {"workload": "plain fine step: 256^3 SDF + 256^3x12 features, rgbnet "
                                "42-192x3-3, 8192 rays, stepsize 0.5, s=256, fwd+bwd+Adam",
                    "rays_per_step": cfg.n_rays, "path": args.path, "mlp": args.mlp},
-        "roofline": {"bound": "hbm", "kernel": roof_kernel, "ms": adam_ms,
-                     "algorithmic_bytes": adam_bytes,
-                     "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic},
+        "roofline": roofline,
+        "k0_adam": {"bound": "hbm", "kernel": adam_kernel, "ms": adam_ms,
+                    "algorithmic_bytes": adam_bytes, "achieved": adam_gbs, "peak": PEAK_HBM_GBS,
+                    "unit": "GB/s", "frac": adam_gbs / PEAK_HBM_GBS},
         "step_roofline": None if sb is None else {
             "algorithmic_bytes": sb, "achieved_GBs": sb / (ms * 1e-3) / 1e9,
             "frac": sb / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS},

@@ -440,6 +440,10 @@ int vx_mlp_layer_fwd(int prec, int kt, int last, const void* in, int in_stride, 
                      const void* wpk, const float* bias, void* out, int out_stride,
                      const float* w3, const float* b3, float* logits, const int64_t* M_dev,
                      void* stream);
+/* The 3-output layer alone: logits [M, 3] = W3 H + b3 from the last hidden activations
+ * H [M, hs] (the last = 1 part of vx_mlp_layer_fwd). */
+int vx_mlp_logits(int prec, const void* H, int hs, int64_t M, const float* w3, const float* b3,
+                  float* logits, const int64_t* M_dev, void* stream);
 /* Data gradient through a layer: dx = (W^T dy) * [mask > 0] (mask NULL: no ReLU). */
 int vx_mlp_layer_bwd(int prec, int nto, const float* dy, int dy_stride, int64_t M,
                      const void* wpk_t, const void* mask, int mask_stride, float* dx,

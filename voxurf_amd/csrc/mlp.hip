@@ -1090,6 +1090,19 @@ int vx_mlp_layer_fwd(int prec, int kt, int last, const void* in, int in_stride, 
   return launch_status("mlp_layer_fwd");
 }
 
+int vx_mlp_logits(int prec, const void* H, int hs, int64_t M, const float* w3, const float* b3,
+                  float* logits, const int64_t* M_dev, void* stream) {
+  VX_CHECK_ARG(prec >= 0 && prec <= 2, "mlp_logits: prec must be 0, 1 or 2");
+  VX_CHECK_ARG(M >= 0, "mlp_logits: M < 0");
+  if (M == 0) return VX_OK;
+  VX_CHECK_ARG(H && w3 && b3 && logits && hs >= kWidth, "mlp_logits: bad arguments");
+  hipStream_t st = (hipStream_t)stream;
+  const unsigned g = grid_for(M, 4, 8 * (int64_t)num_cus());
+  if (prec == 1) k_mlp_logits<true><<<g, 256, 0, st>>>(H, hs, M, M_dev, w3, b3, logits);
+  else k_mlp_logits<false><<<g, 256, 0, st>>>(H, hs, M, M_dev, w3, b3, logits);
+  return launch_status("mlp_logits");
+}
+
 int vx_mlp_layer_bwd(int prec, int nto, const float* dy, int dy_stride, int64_t M,
                      const void* wpk_t, const void* mask, int mask_stride, float* dx,
                      int dx_stride, const int64_t* M_dev, void* stream) {

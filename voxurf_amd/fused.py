@@ -135,6 +135,8 @@ class FusedFineTrainer:
         # a list: (start, end) events of the k0 Adam sweep (phase 1, or the whole pass when
         # not overlapped) of every step are appended to it
         self.adam_events = None
+        # a list: (start, end, rows) of every 192 x 192 rgbnet forward layer launch
+        self.layer_events = None
         self.last_live_pages = None
 
     mask_cache = None
@@ -317,9 +319,11 @@ class FusedFineTrainer:
         m_dev = self.off[n - 1:n] if n > 0 else None
         while True:
             rows = self._rows
+            layer_ev = [] if self.layer_events is not None else None
             if n > 0:
                 Xr = mlp_hip.padded_rows(self._X, din, prec)
-                lg, hs = mlp_hip.forward_rows(Xr, Ws, bs, prec, self._packed, st, M_dev=m_dev)
+                lg, hs = mlp_hip.forward_rows(Xr, Ws, bs, prec, self._packed, st, M_dev=m_dev,
+                                              timing=layer_ev)
             else:
                 lg = torch.zeros((1, 3), dtype=torch.float32, device=self.dev)
             dlogit = torch.empty((max(rows, 1), 3), dtype=torch.float32, device=self.dev)
@@ -346,6 +350,8 @@ class FusedFineTrainer:
             self._grow_rows(m)
             self._gather(n, viewdirs)
         self._set_rows(m)
+        if layer_ev:
+            self.layer_events += [(e0, e1, m) for e0, e1 in layer_ev]
         if profile:
             # snapshot before any optimiser pass clears a gradient
             k0_grad_snapshot = self.k0_grad.clone()

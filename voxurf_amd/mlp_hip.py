@@ -78,10 +78,11 @@ class Packed:
         self.prec = prec
 
 
-def forward_rows(X, Ws, bs, prec, packed, st, M_dev=None):
+def forward_rows(X, Ws, bs, prec, packed, st, M_dev=None, timing=None):
     """logits [M, 3] and the hidden activations of X [M, 32*kt0] (padded rows in the
     activation element type).  M_dev: optional device int64 row count (at most M rows of
-    the buffers are then used), so the launches need not wait for it on the host."""
+    the buffers are then used), so the launches need not wait for it on the host.
+    timing: a list that receives (start, end) events around each 192 x 192 layer."""
     M = X.shape[0]
     dev = X.device
     adt = act_dtype(prec)
@@ -92,9 +93,18 @@ def forward_rows(X, Ws, bs, prec, packed, st, M_dev=None):
     for li in range(nlin - 1):
         last = li == nlin - 2
         H = torch.empty((M, WIDTH), dtype=adt, device=dev)
-        call("vx_mlp_layer_fwd", prec, kt, int(last), ptr(h_in), h_in.shape[1], M,
-             ptr(packed.fwd[li]), ptr(bs[li]), ptr(H), WIDTH, ptr(Ws[-1]) if last else None,
-             ptr(bs[-1]) if last else None, ptr(logits) if last else None, ptr(M_dev), st)
+        timed = timing is not None and li > 0
+        if timed:
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+        call("vx_mlp_layer_fwd", prec, kt, 0, ptr(h_in), h_in.shape[1], M,
+             ptr(packed.fwd[li]), ptr(bs[li]), ptr(H), WIDTH, None, None, None, ptr(M_dev), st)
+        if timed:
+            e1.record()
+            timing.append((e0, e1))
+        if last:
+            call("vx_mlp_logits", prec, ptr(H), WIDTH, M, ptr(Ws[-1]), ptr(bs[-1]), ptr(logits),
+                 ptr(M_dev), st)
         hs.append(H)
         h_in, kt = H, 6
     return logits, hs
