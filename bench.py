@@ -189,12 +189,14 @@ def main():
             adam_bytes = 24 * n_k0 + 8 * live
         # Roofline of the dominant kernel: the rgbnet 192 x 192 hidden layer forward (the
         # layer kernels are the largest share of the step), timed with HIP events around
-        # each launch on its stream over the timed steps; algorithmic FLOPs 2 * 192 * 192
+        # each step's back-to-back run of these launches on their stream over the timed
+        # steps; algorithmic FLOPs 2 * 192 * 192
         # per survivor row, against the FP32 matrix peak (the path computes in FP32; the
         # split kernel executes six BF16 products per FP32 product, so its BF16 MFMA
         # utilisation is also reported).
-        lay_ms = sum(a.elapsed_time(b) for a, b, _ in lev) / len(lev)
-        lay_flops = sum(2.0 * 192 * 192 * m for _, _, m in lev) / len(lev)
+        n_launch = sum(k for _, _, k, _ in lev)
+        lay_ms = sum(a.elapsed_time(b) for a, b, _, _ in lev) / n_launch
+        lay_flops = sum(2.0 * 192 * 192 * m * k for _, _, k, m in lev) / n_launch
         lay_tf = lay_flops / (lay_ms * 1e-3) / 1e12
         prec = trainer.mlp_prec
         peak = 2500.0 if prec == 1 else PEAK_FP32_MFMA_TF

@@ -322,44 +322,61 @@ __global__ void __launch_bounds__(256) k_fused_gather(
 //   loss_r = sum_c (rgb_marched - target)^2,  g = 2 (rgb_marched - target) / (3 N)
 //   dlogit_j = (w_j g) * rgb (1 - rgb);  gw_j = <g, rgb_j>;  glast = <g, bg>;
 //   S_r = sum_j gw_j w_j  (total of the transmittance adjoint's suffix sums)
-__global__ void k_fused_composite(int64_t n_rays, const int64_t* __restrict__ cnt,
-                                  const int64_t* __restrict__ off_incl,
-                                  const float* __restrict__ logits, int lstride,
-                                  const float4* __restrict__ P4, const float* __restrict__ last,
-                                  const float* __restrict__ target, float bg, float inv_count,
-                                  float* __restrict__ rgb_out, float* __restrict__ dlogit,
-                                  float* __restrict__ gw, float* __restrict__ glast,
-                                  float* __restrict__ S, float* __restrict__ loss_r,
-                                  int64_t max_rows) {
-  for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n_rays;
-       r += (int64_t)gridDim.x * blockDim.x) {
+__global__ void __launch_bounds__(256) k_fused_composite(
+    int64_t n_rays, const int64_t* __restrict__ cnt, const int64_t* __restrict__ off_incl,
+    const float* __restrict__ logits, int lstride, const float4* __restrict__ P4,
+    const float* __restrict__ last, const float* __restrict__ target, float bg, float inv_count,
+    float* __restrict__ rgb_out, float* __restrict__ dlogit, float* __restrict__ gw,
+    float* __restrict__ glast, float* __restrict__ S, float* __restrict__ loss_r,
+    int64_t max_rows) {
+  // sixteen lanes per ray (four rays per wavefront): lane sl takes the ray's rows
+  // sl, sl + 16, ...; the per-ray sums close with 16-lane butterflies
+  constexpr int SEG = 16;
+  const int lane = threadIdx.x & 63, grp = lane / SEG, sl = lane % SEG;
+  const int64_t groups = (int64_t)gridDim.x * (blockDim.x >> 6) * (64 / SEG);
+  auto seg_sum = [&](float v) {
+#pragma unroll
+    for (int off = SEG / 2; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+  };
+  for (int64_t r0 = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * (64 / SEG);
+       r0 < n_rays; r0 += groups) {
+    const int64_t r = r0 + grp;
+    const bool ok = r < n_rays;
     // a ray whose rows run past the buffers' capacity is left out (the caller repeats
     // the step with larger buffers)
-    const int64_t c = off_incl[r] <= max_rows ? cnt[r] : 0, o = off_incl[r] - cnt[r];
+    const int64_t c = ok && off_incl[r] <= max_rows ? cnt[r] : 0;
+    const int64_t o = ok ? off_incl[r] - cnt[r] : 0;
     float acc[3] = {0.f, 0.f, 0.f};
-    for (int64_t j = o; j < o + c; ++j) {
+    for (int64_t j = o + sl; j < o + c; j += SEG) {
       const float w = P4[j].w;
       for (int ch = 0; ch < 3; ++ch) {
         const float rgb = 1.f / (1.f + expf(-logits[j * lstride + ch]));
         acc[ch] += w * rgb;
       }
     }
-    float g[3], l = 0.f;
+    for (int ch = 0; ch < 3; ++ch) acc[ch] = seg_sum(acc[ch]);
+    if (!ok) continue;
+    const float lr = last[r];
     if (!target) {  // rendering only
-      for (int ch = 0; ch < 3; ++ch) rgb_out[3 * r + ch] = acc[ch] + last[r] * bg;
+      if (sl == 0)
+        for (int ch = 0; ch < 3; ++ch) rgb_out[3 * r + ch] = acc[ch] + lr * bg;
       continue;
     }
+    float g[3], l = 0.f;
     for (int ch = 0; ch < 3; ++ch) {
-      const float v = acc[ch] + last[r] * bg;
-      rgb_out[3 * r + ch] = v;
+      const float v = acc[ch] + lr * bg;
+      if (sl == 0) rgb_out[3 * r + ch] = v;
       const float d = v - target[3 * r + ch];
       l += d * d;
       g[ch] = 2.f * d * inv_count;
     }
-    loss_r[r] = l;
-    glast[r] = (g[0] + g[1] + g[2]) * bg;
+    if (sl == 0) {
+      loss_r[r] = l;
+      glast[r] = (g[0] + g[1] + g[2]) * bg;
+    }
     float s = 0.f;
-    for (int64_t j = o; j < o + c; ++j) {
+    for (int64_t j = o + sl; j < o + c; j += SEG) {
       const float w = P4[j].w;
       float gwj = 0.f;
       for (int ch = 0; ch < 3; ++ch) {
@@ -370,7 +387,8 @@ __global__ void k_fused_composite(int64_t n_rays, const int64_t* __restrict__ cn
       gw[j] = gwj;
       s += gwj * w;
     }
-    S[r] = s;
+    s = seg_sum(s);
+    if (sl == 0) S[r] = s;
   }
 }
 
@@ -567,7 +585,7 @@ int vx_fused_fine_composite(int64_t n_rays, const int64_t* cnt, const int64_t* o
   VX_CHECK_ARG(n_rays >= 0, "fused_fine_composite: n_rays < 0");
   if (n_rays == 0) return VX_OK;
   const float inv_count = 1.f / (3.f * (float)n_rays);
-  k_fused_composite<<<grid_for(n_rays, 64), 64, 0, (hipStream_t)stream>>>(
+  k_fused_composite<<<grid_for(n_rays, 16, 1 << 16), 256, 0, (hipStream_t)stream>>>(
       n_rays, cnt, off_incl, logits, lstride, (const float4*)P4, last, target, bg, inv_count, rgb,
       dlogit, gw, glast, S, loss_r, max_rows);
   return launch_status("fused_fine_composite");

@@ -629,44 +629,66 @@ __global__ void __launch_bounds__(kLayerThreads) k_mlp_layer_split(
   }
 }
 
-// logits [M, 3] = H W3^T + b3 from the stored activations (bfloat16 when BF).  One wave
-// per row: lanes 0..47 take four features each, then a wave sum.
+// logits [M, 3] = H W3^T + b3 from the stored activations (bfloat16 when BF).  Sixteen
+// lanes per row (three float4 of features each, 256 contiguous bytes per load across the
+// group), four rows per wave and two such row sets in flight per iteration; the partial
+// dot products are summed over the 16-lane group.
 template <bool BF>
 __global__ void __launch_bounds__(256) k_mlp_logits(const void* __restrict__ H_, int hs,
                                                     int64_t M, const int64_t* __restrict__ Md,
                                                     const float* __restrict__ w3,
                                                     const float* __restrict__ b3,
                                                     float* __restrict__ logits) {
+  constexpr int U = 2;
   M = eff_rows(M, Md);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const bool act = lane < kWidth / 4;
-  const int f = act ? 4 * lane : 0;
-  float4 w[3];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, rg = lane >> 4, sl = lane & 15;
+  float4 w[3][3];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    w[c] = *reinterpret_cast<const float4*>(w3 + c * kWidth + f);
-    if (!act) w[c] = make_float4(0.f, 0.f, 0.f, 0.f);
-  }
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+      w[c][q] = *reinterpret_cast<const float4*>(w3 + c * kWidth + 64 * q + 4 * sl);
   const float c0 = b3[0], c1 = b3[1], c2 = b3[2];
-  for (int64_t r = (int64_t)blockIdx.x * 4 + wave; r < M; r += (int64_t)gridDim.x * 4) {
-    float4 x;
-    if constexpr (BF) {
-      const uint2 q = *reinterpret_cast<const uint2*>(reinterpret_cast<const __bf16*>(H_) + r * hs + f);
-      const __bf16* b = reinterpret_cast<const __bf16*>(&q);
-      x = make_float4((float)b[0], (float)b[1], (float)b[2], (float)b[3]);
-    } else {
-      x = *reinterpret_cast<const float4*>(reinterpret_cast<const float*>(H_) + r * hs + f);
-    }
-    float p[3];
+  const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+  for (int64_t r0 = ((int64_t)blockIdx.x * (blockDim.x >> 6) + wave) * 4 * U; r0 < M;
+       r0 += waves * 4 * U) {
+    float4 x[U][3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      p[c] = act ? x.x * w[c].x + x.y * w[c].y + x.z * w[c].z + x.w * w[c].w : 0.f;
-      p[c] = wave_sum_f32(p[c]);
+    for (int u = 0; u < U; ++u) {
+      const int64_t r = r0 + 4 * u + rg;
+      const int64_t rr = r < M ? r : M - 1;
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const int f = 64 * q + 4 * sl;
+        if constexpr (BF) {
+          const uint2 v = *reinterpret_cast<const uint2*>(reinterpret_cast<const __bf16*>(H_) + rr * hs + f);
+          const __bf16* b = reinterpret_cast<const __bf16*>(&v);
+          x[u][q] = make_float4((float)b[0], (float)b[1], (float)b[2], (float)b[3]);
+        } else {
+          x[u][q] = *reinterpret_cast<const float4*>(reinterpret_cast<const float*>(H_) + rr * hs + f);
+        }
+      }
     }
-    if (lane == 0) {
-      logits[3 * r] = p[0] + c0;
-      logits[3 * r + 1] = p[1] + c1;
-      logits[3 * r + 2] = p[2] + c2;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      float p[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        float t = 0.f;
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+          t += x[u][q].x * w[c][q].x + x[u][q].y * w[c][q].y + x[u][q].z * w[c][q].z +
+               x[u][q].w * w[c][q].w;
+#pragma unroll
+        for (int off = 8; off >= 1; off >>= 1) t += __shfl_xor(t, off, 64);
+        p[c] = t;
+      }
+      const int64_t r = r0 + 4 * u + rg;
+      if (sl == 0 && r < M) {
+        logits[3 * r] = p[0] + c0;
+        logits[3 * r + 1] = p[1] + c1;
+        logits[3 * r + 2] = p[2] + c2;
+      }
     }
   }
 }
@@ -1083,7 +1105,7 @@ int vx_mlp_layer_fwd(int prec, int kt, int last, const void* in, int in_stride, 
 #undef L
 #undef LS
   if (last) {
-    const unsigned g = grid_for(M, 4, 8 * (int64_t)num_cus());
+    const unsigned g = grid_for(M, 32, 8 * (int64_t)num_cus());
     if (bf16) k_mlp_logits<true><<<g, 256, 0, st>>>(out, out_stride, M, M_dev, w3, b3, logits);
     else k_mlp_logits<false><<<g, 256, 0, st>>>(out, out_stride, M, M_dev, w3, b3, logits);
   }
@@ -1097,7 +1119,7 @@ int vx_mlp_logits(int prec, const void* H, int hs, int64_t M, const float* w3, c
   if (M == 0) return VX_OK;
   VX_CHECK_ARG(H && w3 && b3 && logits && hs >= kWidth, "mlp_logits: bad arguments");
   hipStream_t st = (hipStream_t)stream;
-  const unsigned g = grid_for(M, 4, 8 * (int64_t)num_cus());
+  const unsigned g = grid_for(M, 32, 8 * (int64_t)num_cus());
   if (prec == 1) k_mlp_logits<true><<<g, 256, 0, st>>>(H, hs, M, M_dev, w3, b3, logits);
   else k_mlp_logits<false><<<g, 256, 0, st>>>(H, hs, M, M_dev, w3, b3, logits);
   return launch_status("mlp_logits");

@@ -135,7 +135,7 @@ class FusedFineTrainer:
         # a list: (start, end) events of the k0 Adam sweep (phase 1, or the whole pass when
         # not overlapped) of every step are appended to it
         self.adam_events = None
-        # a list: (start, end, rows) of every 192 x 192 rgbnet forward layer launch
+        # a list: (start, end, launches, rows) around the 192 x 192 rgbnet forward layers
         self.layer_events = None
         self.last_live_pages = None
 
@@ -351,7 +351,7 @@ class FusedFineTrainer:
             self._gather(n, viewdirs)
         self._set_rows(m)
         if layer_ev:
-            self.layer_events += [(e0, e1, m) for e0, e1 in layer_ev]
+            self.layer_events += [(e0, e1, k, m) for e0, e1, k in layer_ev]
         if profile:
             # snapshot before any optimiser pass clears a gradient
             k0_grad_snapshot = self.k0_grad.clone()

@@ -82,7 +82,8 @@ def forward_rows(X, Ws, bs, prec, packed, st, M_dev=None, timing=None):
     """logits [M, 3] and the hidden activations of X [M, 32*kt0] (padded rows in the
     activation element type).  M_dev: optional device int64 row count (at most M rows of
     the buffers are then used), so the launches need not wait for it on the host.
-    timing: a list that receives (start, end) events around each 192 x 192 layer."""
+    timing: a list that receives (start, end, launches): events around the run of
+    192 x 192 layers (back to back on the stream) and their number."""
     M = X.shape[0]
     dev = X.device
     adt = act_dtype(prec)
@@ -93,15 +94,15 @@ def forward_rows(X, Ws, bs, prec, packed, st, M_dev=None, timing=None):
     for li in range(nlin - 1):
         last = li == nlin - 2
         H = torch.empty((M, WIDTH), dtype=adt, device=dev)
-        timed = timing is not None and li > 0
-        if timed:
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        if timing is not None and li == 1:
+            e0 = torch.cuda.Event(enable_timing=True)
             e0.record()
         call("vx_mlp_layer_fwd", prec, kt, 0, ptr(h_in), h_in.shape[1], M,
              ptr(packed.fwd[li]), ptr(bs[li]), ptr(H), WIDTH, None, None, None, ptr(M_dev), st)
-        if timed:
+        if timing is not None and li == nlin - 2 and li >= 1:
+            e1 = torch.cuda.Event(enable_timing=True)
             e1.record()
-            timing.append((e0, e1))
+            timing.append((e0, e1, nlin - 2))
         if last:
             call("vx_mlp_logits", prec, ptr(H), WIDTH, M, ptr(Ws[-1]), ptr(bs[-1]), ptr(logits),
                  ptr(M_dev), st)
