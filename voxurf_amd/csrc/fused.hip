@@ -134,7 +134,7 @@ __device__ __forceinline__ int kth_set_bit(uint64_t m, int k) {
 // {m0 processed, m1 in the walk range (= slots used), m0 of the whole ray (if count_all)};
 // per walk-range sample k of ray r at slot r*cap + k: s4 = (x, y, z, T before the
 // sample), g4 = (gx, gy, gz, sdf).
-__global__ void __launch_bounds__(256) k_fused_fwd(
+__global__ void __launch_bounds__(256, 3) k_fused_fwd(
     FusedGeom G, const float* __restrict__ rays_o, const float* __restrict__ rays_d,
     const float* __restrict__ viewdirs, int64_t n_rays, int cap, float4* __restrict__ s4,
     float4* __restrict__ g4, int64_t* __restrict__ cnt_out, float* __restrict__ last_out,

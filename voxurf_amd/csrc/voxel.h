@@ -365,8 +365,8 @@ __device__ __forceinline__ void sdf7(const T* __restrict__ grid, const GridDesc&
                                      const T* mn, const T* mx, T voxel_size, T& sdf, T grad[3],
                                      T* feat) {
 #pragma clang fp contract(off)
-  // the seven cells first, then all 56 corner loads, then the sums: one memory latency
-  // per sample instead of seven
+  // the seven cells first, then all corner loads, then the sums: one memory latency per
+  // sample instead of seven
   Tri<T> t[7];
   t[0].init(world_to_index(q[0], mn[0], mx[0] - mn[0], g.X),
             world_to_index(q[1], mn[1], mx[1] - mn[1], g.Y),
@@ -378,15 +378,90 @@ __device__ __forceinline__ void sdf7(const T* __restrict__ grid, const GridDesc&
     displaced_cell(g, ind, a, (T)-1, t[1 + 2 * a]);
     displaced_cell(g, ind, a, (T)1, t[2 + 2 * a]);
   }
-  T v[7][8];
-  unsigned in[7];
+  // In the interior a displaced cell is the centre cell shifted by one voxel along its
+  // axis (or, where the index was clamped at a face, the centre cell itself), so the
+  // seven cells cover 32 distinct voxels: the centre's 8 and, per axis and side, the 4
+  // beyond it.  When that holds for every lane, only those 32 are loaded and each cell's
+  // 8 corner values are selected from them (the same values, weights and sums as 56
+  // separate loads); otherwise every cell loads its own corners.
+  int dd[6];
+  bool shared = true;
 #pragma unroll
-  for (int i = 0; i < 7; ++i) tri_load(grid, g, t[i], 0, v[i], in[i]);
-  sdf = tri_sum(t[0], v[0], in[0]);
+  for (int i = 1; i < 7; ++i) {
+    const int c = 2 - (i - 1) / 2, s = (i - 1) % 2 ? 1 : -1;  // world axis, side
+    const int d[3] = {t[i].x0 - t[0].x0, t[i].y0 - t[0].y0, t[i].z0 - t[0].z0};
+    dd[i - 1] = d[c];
+    shared = shared && d[(c + 1) % 3] == 0 && d[(c + 2) % 3] == 0 && (d[c] == 0 || d[c] == s);
+  }
+  T f[7];  // the seven lookups: centre, then -/+ along z, y, x of the zyx index
+  if (__all(shared)) {
+    T v0[8], vo[3][2][4];
+    unsigned in0, ino[3][2];
+    tri_load(grid, g, t[0], 0, v0, in0);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int side = 0; side < 2; ++side) {
+        const int lo = (c + 1) % 3 < (c + 2) % 3 ? (c + 1) % 3 : (c + 2) % 3;
+        const int hi = (c + 1) % 3 < (c + 2) % 3 ? (c + 2) % 3 : (c + 1) % 3;
+        ino[c][side] = 0u;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          // the voxel beyond the centre cell along axis c; j holds the corner bits of
+          // the other two axes (the lower-numbered axis first)
+          int o[3];
+          o[c] = side ? 2 : -1;
+          o[lo] = j >> 1;
+          o[hi] = j & 1;
+          const int x = t[0].x0 + o[0], y = t[0].y0 + o[1], z = t[0].z0 + o[2];
+          if (in_grid<T>(x, y, z, g)) ino[c][side] |= 1u << j;
+          vo[c][side][j] = grid[clamp_index(x, g.X) * g.sx + clamp_index(y, g.Y) * g.sy +
+                                clamp_index(z, g.Z) * g.sz];
+        }
+      }
+    f[0] = tri_sum(t[0], v0, in0);
+#pragma unroll
+    for (int i = 1; i < 7; ++i) {
+      const int c = 2 - (i - 1) / 2, side = (i - 1) % 2, s = side ? 1 : -1;
+      const int lo = (c + 1) % 3 < (c + 2) % 3 ? (c + 1) % 3 : (c + 2) % 3;
+      const int hi = (c + 1) % 3 < (c + 2) % 3 ? (c + 2) % 3 : (c + 1) % 3;
+      const bool same = dd[i - 1] == 0;
+      T vi[8];
+      unsigned m = 0u;
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        const int bb[3] = {b >> 2, (b >> 1) & 1, b & 1};
+        const int cb = bb[c] + s;  // the corner's index along c in the centre cell when shifted
+        T val;
+        bool inb;
+        if (cb == 0 || cb == 1) {
+          const int k = b ^ ((bb[c] ^ cb) << (2 - c));
+          val = v0[k];
+          inb = (in0 >> k) & 1u;
+        } else {
+          const int j = bb[lo] * 2 + bb[hi];
+          val = vo[c][side][j];
+          inb = (ino[c][side] >> j) & 1u;
+        }
+        // not shifted (clamped at a face): the centre cell's own corner b
+        vi[b] = same ? v0[b] : val;
+        if (same ? ((in0 >> b) & 1u) : inb) m |= 1u << b;
+      }
+      f[i] = tri_sum(t[i], vi, m);
+    }
+  } else {
+    T v[7][8];
+    unsigned in[7];
+#pragma unroll
+    for (int i = 0; i < 7; ++i) tri_load(grid, g, t[i], 0, v[i], in[i]);
+#pragma unroll
+    for (int i = 0; i < 7; ++i) f[i] = tri_sum(t[i], v[i], in[i]);
+  }
+  sdf = f[0];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const T fm = tri_sum(t[1 + 2 * a], v[1 + 2 * a], in[1 + 2 * a]);
-    const T fp = tri_sum(t[2 + 2 * a], v[2 + 2 * a], in[2 + 2 * a]);
+    const T fm = f[1 + 2 * a];
+    const T fp = f[2 + 2 * a];
     const T df = clamped_diff(ind, a, (T)1, g);
     grad[2 - a] = ((fp - fm) / df) / voxel_size;
     if (feat) {
