@@ -505,6 +505,10 @@ __global__ void __launch_bounds__(256) k_fused_bwd(
       }
       const float back = base_back - (prefix + incl);
       prefix += __shfl(incl, SEG - 1, SEG);
+      // The SDF adjoint of this sample as terms on the 4x4x4 voxel block around its cell;
+      // consecutive samples of the ray in the same cell (a run of lanes) sum their terms
+      // and the run's first lane issues the atomics: one set of adds per visited cell.
+      Sdf7Terms<float> tm;
       if (valid) {
         const double den = (double)(1.f - ss.alpha) + 1e-10;
         const float da = (float)((double)(gwl * ss.myT) - (double)back / den);
@@ -513,8 +517,44 @@ __global__ void __launch_bounds__(256) k_fused_bwd(
         gg[0] += dg[0];
         gg[1] += dg[1];
         gg[2] += dg[2];
-        sdf7_scatter<float>(sdf_grad, G.gs, ss.q, G.mn, G.mx, G.voxel_size, gsdf, gg, nullptr);
+        sdf7_terms<float>(sdf_grad, G.gs, ss.q, G.mn, G.mx, G.voxel_size, gsdf, gg, tm);
+      } else {
+        tm.x0 = -2 - lane;  // matches no cell
+        tm.y0 = tm.z0 = 0;
+#pragma unroll
+        for (int b = 0; b < 8; ++b) tm.acc[b] = 0.f;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int sd = 0; sd < 2; ++sd)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) tm.out[c][sd][j] = 0.f;
       }
+      const int px = __shfl_up(tm.x0, 1, SEG), py = __shfl_up(tm.y0, 1, SEG),
+                pz = __shfl_up(tm.z0, 1, SEG);
+      const bool head = sl == 0 || px != tm.x0 || py != tm.y0 || pz != tm.z0;
+      const uint64_t hm = (__ballot(head) >> (grp * SEG)) & 0xFFFFull;
+      const int rid = __popcll(hm & ((2ull << sl) - 1ull));  // run index of this lane
+#pragma unroll
+      for (int off = 1; off < SEG; off <<= 1) {
+        const int orid = __shfl_down(rid, off, SEG);
+        const bool same = sl + off < SEG && orid == rid;
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+          const float v = __shfl_down(tm.acc[b], off, SEG);
+          if (same) tm.acc[b] += v;
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int sd = 0; sd < 2; ++sd)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              const float v = __shfl_down(tm.out[c][sd][j], off, SEG);
+              if (same) tm.out[c][sd][j] += v;
+            }
+      }
+      if (valid && head) sdf7_emit<float>(sdf_grad, G.gs, tm);
     }
   }
 }

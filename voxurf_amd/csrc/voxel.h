@@ -536,4 +536,106 @@ __device__ __forceinline__ void sdf7_scatter(T* __restrict__ ggrid, const GridDe
 }
 
 
+// sdf7_scatter in two halves, so that consecutive samples of a ray in the same cell can
+// add their terms before the atomics.  sdf7_terms: the 56 corner terms of one sample
+// (centre, then -/+ one voxel along z, y, x of the zyx index) summed per voxel of the
+// 4x4x4 block around the centre cell that the seven cells can reach: acc[8] for the
+// centre cell's corners, out[c][side][j] for the four voxels beyond it along world axis
+// c (side 0: below, 1: above; j the corner bits of the other two axes, lower-numbered
+// axis first).  A displaced cell that is neither the centre cell shifted by one voxel
+// along its axis nor the centre cell itself (clamped at a face) is scattered directly.
+// sdf7_emit adds a block's terms to the gradient grid.
+template <typename T>
+struct Sdf7Terms {
+  int x0, y0, z0;
+  T acc[8];
+  T out[3][2][4];
+};
+
+template <typename T>
+__device__ __forceinline__ void sdf7_terms(T* __restrict__ ggrid, const GridDesc& g, const T q[3],
+                                           const T* mn, const T* mx, T voxel_size, T gsdf,
+                                           const T ggrad[3], Sdf7Terms<T>& o) {
+#pragma clang fp contract(off)
+  Tri<T> t0;
+  t0.init(world_to_index(q[0], mn[0], mx[0] - mn[0], g.X),
+          world_to_index(q[1], mn[1], mx[1] - mn[1], g.Y),
+          world_to_index(q[2], mn[2], mx[2] - mn[2], g.Z));
+  o.x0 = t0.x0;
+  o.y0 = t0.y0;
+  o.z0 = t0.z0;
+#pragma unroll
+  for (int b = 0; b < 8; ++b) o.acc[b] = gsdf != (T)0 ? t0.w(b >> 2, (b >> 1) & 1, b & 1) * gsdf : (T)0;
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int sd = 0; sd < 2; ++sd)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) o.out[c][sd][j] = (T)0;
+  T ind[3];
+  zyx_index(q, mn, mx, g, ind);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int c = 2 - a;  // world axis of zyx axis a (0 = x)
+    const int lo = (c + 1) % 3 < (c + 2) % 3 ? (c + 1) % 3 : (c + 2) % 3;
+    const int hi = (c + 1) % 3 < (c + 2) % 3 ? (c + 2) % 3 : (c + 1) % 3;
+    const T df = clamped_diff(ind, a, (T)1, g);
+    const T gd = (ggrad[c] / df) / voxel_size;
+    const T gm = -gd, gp = gd;
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      const int s = side ? 1 : -1;
+      const T go = side ? gp : gm;
+      if (go == (T)0) continue;
+      Tri<T> td;
+      displaced_cell(g, ind, a, (T)s, td);
+      const int d[3] = {td.x0 - t0.x0, td.y0 - t0.y0, td.z0 - t0.z0};
+      const bool others = d[lo] == 0 && d[hi] == 0;
+      if (others && d[c] == 0) {
+#pragma unroll
+        for (int b = 0; b < 8; ++b) o.acc[b] += td.w(b >> 2, (b >> 1) & 1, b & 1) * go;
+      } else if (others && d[c] == s) {
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+          const int bb[3] = {b >> 2, (b >> 1) & 1, b & 1};
+          const T w = td.w(bb[0], bb[1], bb[2]) * go;
+          const int cb = bb[c] + s;  // this corner's index along c in the centre cell
+          if (cb == 0 || cb == 1)
+            o.acc[b ^ ((bb[c] ^ cb) << (2 - c))] += w;
+          else
+            o.out[c][side][bb[lo] * 2 + bb[hi]] += w;
+        }
+      } else {
+        tri_scatter(ggrid, g, td, 0, go);
+      }
+    }
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void sdf7_emit(T* __restrict__ ggrid, const GridDesc& g,
+                                          const Sdf7Terms<T>& o) {
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const int x = o.x0 + (b >> 2), y = o.y0 + ((b >> 1) & 1), z = o.z0 + (b & 1);
+    if (o.acc[b] != (T)0 && in_grid<T>(x, y, z, g)) atomic_add(&ggrid[x * g.sx + y * g.sy + z * g.sz], o.acc[b]);
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const int lo = (c + 1) % 3 < (c + 2) % 3 ? (c + 1) % 3 : (c + 2) % 3;
+    const int hi = (c + 1) % 3 < (c + 2) % 3 ? (c + 2) % 3 : (c + 1) % 3;
+#pragma unroll
+    for (int sd = 0; sd < 2; ++sd)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        int v[3] = {o.x0, o.y0, o.z0};
+        v[c] += sd ? 2 : -1;
+        v[lo] += j >> 1;
+        v[hi] += j & 1;
+        if (o.out[c][sd][j] != (T)0 && in_grid<T>(v[0], v[1], v[2], g))
+          atomic_add(&ggrid[v[0] * g.sx + v[1] * g.sy + v[2] * g.sz], o.out[c][sd][j]);
+      }
+  }
+}
+
 }  // namespace vx
