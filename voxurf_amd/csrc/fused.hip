@@ -408,17 +408,53 @@ __global__ void __launch_bounds__(256) k_fused_feat_bwd(
     // Trilinear scatter of the feature adjoint: four samples per pass, sixteen lanes
     // per sample, lane cl taking channels cl, cl + 16, ...  With channels innermost the
     // lanes of one atomic instruction hit one or two cache lines per sample instead of
-    // one line per lane.
+    // one line per lane.  The four samples are consecutive rows (survivors of a ray in
+    // depth order): those in the same cell sum their corner terms first, and the first
+    // of them issues the atomics.
     for (int i = 0; i < 16; ++i) {
+      if (base + 4 * i >= m) break;  // uniform over the wavefront
       const int64_t j = base + 4 * i + grp;
-      if (j >= m) break;
-      const float4 a = P4[j];
+      const bool valid = j < m;
+      const float4 a = P4[valid ? j : m - 1];
       Tri<float> t;
       t.init(world_to_index(a.x, G.mn[0], G.mx[0] - G.mn[0], G.gk.X),
              world_to_index(a.y, G.mn[1], G.mx[1] - G.mn[1], G.gk.Y),
              world_to_index(a.z, G.mn[2], G.mx[2] - G.mn[2], G.gk.Z));
-      for (int ch = cl; ch < C; ch += 16) tri_scatter(k0_grad, G.gk, t, ch, dX[j * dxs + ch]);
-      if (pages && cl < 8) mark_corner_pages(pages, G.gk, t, cl);
+      if (!valid) t.x0 = -2 - grp;  // matches no cell
+      const int px = __shfl(t.x0, (lane - 16) & 63, 64), py = __shfl(t.y0, (lane - 16) & 63, 64),
+                pz = __shfl(t.z0, (lane - 16) & 63, 64);
+      const bool head = grp == 0 || px != t.x0 || py != t.y0 || pz != t.z0;
+      const uint64_t hb = __ballot(head);
+      // run index of this sample among the four (head flags of groups 0..grp)
+      int rid = 0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) rid += q <= grp && ((hb >> (16 * q)) & 1ull) ? 1 : 0;
+      for (int ch0 = 0; ch0 < C; ch0 += 16) {
+        const int ch = ch0 + cl;
+        const float go = valid && ch < C ? dX[j * dxs + ch] : 0.f;
+        float v[8];
+#pragma unroll
+        for (int b = 0; b < 8; ++b) v[b] = t.w(b >> 2, (b >> 1) & 1, b & 1) * go;
+#pragma unroll
+        for (int off = 16; off < 64; off <<= 1) {
+          const int orid = __shfl_down(rid, off, 64);
+          const bool same = grp + off / 16 < 4 && orid == rid;
+#pragma unroll
+          for (int b = 0; b < 8; ++b) {
+            const float o = __shfl_down(v[b], off, 64);
+            if (same) v[b] += o;
+          }
+        }
+        if (valid && head && ch < C) {
+#pragma unroll
+          for (int b = 0; b < 8; ++b) {
+            const int x = t.x0 + (b >> 2), y = t.y0 + ((b >> 1) & 1), z = t.z0 + (b & 1);
+            if (v[b] != 0.f && in_grid<float>(x, y, z, G.gk))
+              atomic_add(&k0_grad[ch * G.gk.sc + x * G.gk.sx + y * G.gk.sy + z * G.gk.sz], v[b]);
+          }
+        }
+      }
+      if (valid && pages && cl < 8) mark_corner_pages(pages, G.gk, t, cl);
     }
     // adjoint of the normal n = g / (|g| + 1e-5): one sample per lane
     const int64_t j = base + lane;
