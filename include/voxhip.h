@@ -435,19 +435,24 @@ int64_t vx_mlp_pack_elems(int kt, int nto);
 int64_t vx_mlp_pack_net_elems(int n_hidden, int k_in0);
 int vx_mlp_pack_net(int prec, int n_hidden, const int64_t* w_ptrs, int k_in0, void* out,
                     void* stream);
-/* One hidden layer: out = relu(W in + b); with last = 1 also logits [M,3] = W3 out + b3. */
+/* One hidden layer: out = relu(W in + b); with last = 1 also logits [M,3] = W3 out + b3.
+ * mask_bits (prec 2 only, may be NULL): also writes the ReLU mask as bits, six uint32
+ * words per row (bit k of word w: out column 32 w + k positive), for vx_mlp_layer_bwd. */
 int vx_mlp_layer_fwd(int prec, int kt, int last, const void* in, int in_stride, int64_t M,
                      const void* wpk, const float* bias, void* out, int out_stride,
                      const float* w3, const float* b3, float* logits, const int64_t* M_dev,
-                     void* stream);
+                     uint32_t* mask_bits, void* stream);
 /* The 3-output layer alone: logits [M, 3] = W3 H + b3 from the last hidden activations
  * H [M, hs] (the last = 1 part of vx_mlp_layer_fwd). */
 int vx_mlp_logits(int prec, const void* H, int hs, int64_t M, const float* w3, const float* b3,
                   float* logits, const int64_t* M_dev, void* stream);
-/* Data gradient through a layer: dx = (W^T dy) * [mask > 0] (mask NULL: no ReLU). */
+/* Data gradient through a layer: dx = (W^T dy) * [mask > 0] (mask NULL: no ReLU).
+ * mask_bits (prec 2, nto 6, may be NULL): the mask as written by vx_mlp_layer_fwd, read
+ * instead of the float rows (mask must still be given). */
 int vx_mlp_layer_bwd(int prec, int nto, const float* dy, int dy_stride, int64_t M,
                      const void* wpk_t, const void* mask, int mask_stride, float* dx,
-                     int dx_stride, const int64_t* M_dev, void* stream);
+                     int dx_stride, const int64_t* M_dev, const uint32_t* mask_bits,
+                     void* stream);
 /* Floats the `part` / `partb` workspaces of vx_mlp_wgrad and vx_mlp_last_bwd need for
  * M rows (row-block partial sums plus the reduction's group sums). */
 int64_t vx_mlp_wgrad_part_elems(int64_t M);

@@ -448,17 +448,23 @@ __device__ __forceinline__ void load_slab(float4 (&x)[4], const float* __restric
 }
 
 // Epilogue of k_mlp_layer's modes on float32 rows: bias and ReLU (MODE 0), the ReLU mask
-// (MODE 2), or the plain product (MODE 3); columns col0 + 32 to + ...
+// (MODE 2), or the plain product (MODE 3); columns col0 + 32 to + ...  bits (may be
+// NULL): the ReLU masks as bits, six words per 192-column row (bit k of word w: column
+// 32 w + k positive) -- written by MODE 0, read by MODE 2 instead of the float mask.
 template <int NTL, int MODE, bool FULL>
 __device__ __forceinline__ void split_store(const f32x16 (&acc)[NTL], const float* lb,
                                             const float* __restrict__ mask, int mask_stride,
+                                            uint32_t* __restrict__ bits,
                                             float* __restrict__ out, int out_stride, int64_t row,
                                             int h, int to_lo, int to_hi, int col0) {
   const int64_t obase = row * out_stride + 4 * h;
   const int64_t mbase = row * mask_stride + 4 * h;
+  uint32_t* brow = bits ? bits + row * (kWidth / 32) + (col0 >> 5) : nullptr;
 #pragma unroll
   for (int to = 0; to < NTL; ++to) {
     if (!FULL && (to < to_lo || to >= to_hi)) continue;
+    uint32_t word = 0u;
+    if (MODE == 2 && brow) word = brow[to];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       float v4[4];
@@ -473,14 +479,29 @@ __device__ __forceinline__ void split_store(const f32x16 (&acc)[NTL], const floa
         v4[w] = x;
       }
       const int cofs = col0 + 32 * to + 8 * u;
+      const int bit0 = 8 * u + 4 * h;  // bit of column cofs + 4 h in the word
+      if (MODE == 0 && brow) {
+#pragma unroll
+        for (int w = 0; w < 4; ++w) word |= (v4[w] > 0.f ? 1u : 0u) << (bit0 + w);
+      }
       if (MODE == 2) {
-        const float4 mk = *reinterpret_cast<const float4*>(mask + mbase + cofs);
-        v4[0] = mk.x > 0.f ? v4[0] : 0.f;
-        v4[1] = mk.y > 0.f ? v4[1] : 0.f;
-        v4[2] = mk.z > 0.f ? v4[2] : 0.f;
-        v4[3] = mk.w > 0.f ? v4[3] : 0.f;
+        if (brow) {
+#pragma unroll
+          for (int w = 0; w < 4; ++w) v4[w] = (word >> (bit0 + w)) & 1u ? v4[w] : 0.f;
+        } else {
+          const float4 mk = *reinterpret_cast<const float4*>(mask + mbase + cofs);
+          v4[0] = mk.x > 0.f ? v4[0] : 0.f;
+          v4[1] = mk.y > 0.f ? v4[1] : 0.f;
+          v4[2] = mk.z > 0.f ? v4[2] : 0.f;
+          v4[3] = mk.w > 0.f ? v4[3] : 0.f;
+        }
       }
       *reinterpret_cast<float4*>(out + obase + cofs) = make_float4(v4[0], v4[1], v4[2], v4[3]);
+    }
+    if (MODE == 0 && brow) {
+      // the two lane halves hold the two halves of the row's 32 columns
+      word |= (uint32_t)__shfl_xor((int)word, 32, 64);
+      if (h == 0) brow[to] = word;
     }
   }
 }
@@ -493,9 +514,9 @@ template <int KT, int NTL, int MODE, bool FULL>
 __device__ __forceinline__ void split_tile(float4 (&ring)[3][4], const float* __restrict__ in,
                                            int in_stride, int64_t M, const bf16x8* lw,
                                            const float* lb, const float* __restrict__ mask,
-                                           int mask_stride, float* __restrict__ out,
-                                           int out_stride, int64_t tile, int64_t next_row,
-                                           int to_lo, int to_hi, int col0) {
+                                           int mask_stride, uint32_t* __restrict__ bits,
+                                           float* __restrict__ out, int out_stride, int64_t tile,
+                                           int64_t next_row, int to_lo, int to_hi, int col0) {
   static_assert(KT % 3 == 0, "ring of three slabs");
   const int lane = threadIdx.x & 63, h = lane >> 5, rl = lane & 31;
   const int64_t row = tile * 32 + rl;
@@ -512,7 +533,7 @@ __device__ __forceinline__ void split_tile(float4 (&ring)[3][4], const float* __
     split_slab<KT, NTL, FULL>(acc, lw, lane, ring[t % 3], t, to_lo, to_hi);
   }
   if (row < M)
-    split_store<NTL, MODE, FULL>(acc, lb, mask, mask_stride, out, out_stride, row, h, to_lo,
+    split_store<NTL, MODE, FULL>(acc, lb, mask, mask_stride, bits, out, out_stride, row, h, to_lo,
                                  to_hi, col0);
 }
 
@@ -522,9 +543,9 @@ template <int NTL, int MODE, bool FULL>
 __device__ __forceinline__ void split_tile2(float4 (&ring)[4][4], const float* __restrict__ in,
                                             int in_stride, int64_t M, const bf16x8* lw,
                                             const float* lb, const float* __restrict__ mask,
-                                            int mask_stride, float* __restrict__ out,
-                                            int out_stride, int64_t tile, int64_t next_row,
-                                            int to_lo, int to_hi, int col0) {
+                                            int mask_stride, uint32_t* __restrict__ bits,
+                                            float* __restrict__ out, int out_stride, int64_t tile,
+                                            int64_t next_row, int to_lo, int to_hi, int col0) {
   const int lane = threadIdx.x & 63, h = lane >> 5, rl = lane & 31;
   const int64_t row = tile * 32 + rl;
   f32x16 acc[NTL];
@@ -544,7 +565,7 @@ __device__ __forceinline__ void split_tile2(float4 (&ring)[4][4], const float* _
     ring[1][u] = ring[3][u];
   }
   if (row < M)
-    split_store<NTL, MODE, FULL>(acc, lb, mask, mask_stride, out, out_stride, row, h, to_lo,
+    split_store<NTL, MODE, FULL>(acc, lb, mask, mask_stride, bits, out, out_stride, row, h, to_lo,
                                  to_hi, col0);
 }
 
@@ -552,7 +573,8 @@ template <int KT, int NTL, int NCG, int MODE>
 __global__ void __launch_bounds__(kLayerThreads) k_mlp_layer_split(
     const float* __restrict__ in, int in_stride, int64_t M, const int64_t* __restrict__ Md,
     const bf16x8* __restrict__ wpk, const float* __restrict__ bias,
-    const float* __restrict__ mask, int mask_stride, float* __restrict__ out, int out_stride) {
+    const float* __restrict__ mask, int mask_stride, float* __restrict__ out, int out_stride,
+    uint32_t* __restrict__ bits) {
   static_assert(KT == 2 || KT % 3 == 0, "input width 64, 96 or 192");
   constexpr int S = 2 * KT;
   constexpr int WV = NTL * S * 3 * 64;   // bf16x8 vectors of one column group
@@ -612,17 +634,17 @@ __global__ void __launch_bounds__(kLayerThreads) k_mlp_layer_split(
     const int64_t next_row = un < u1 ? lane_row(un / NTL) : -1;
     if constexpr (KT == 2) {
       if (lo == 0 && hi == NTL)
-        split_tile2<NTL, MODE, true>(ring, in, in_stride, M, lw, lb, mask, mask_stride, out,
+        split_tile2<NTL, MODE, true>(ring, in, in_stride, M, lw, lb, mask, mask_stride, bits, out,
                                      out_stride, tile, next_row, 0, NTL, col0);
       else
-        split_tile2<NTL, MODE, false>(ring, in, in_stride, M, lw, lb, mask, mask_stride, out,
+        split_tile2<NTL, MODE, false>(ring, in, in_stride, M, lw, lb, mask, mask_stride, bits, out,
                                       out_stride, tile, next_row, lo, hi, col0);
     } else {
       if (lo == 0 && hi == NTL)
-        split_tile<KT, NTL, MODE, true>(ring, in, in_stride, M, lw, lb, mask, mask_stride, out,
+        split_tile<KT, NTL, MODE, true>(ring, in, in_stride, M, lw, lb, mask, mask_stride, bits, out,
                                         out_stride, tile, next_row, 0, NTL, col0);
       else
-        split_tile<KT, NTL, MODE, false>(ring, in, in_stride, M, lw, lb, mask, mask_stride, out,
+        split_tile<KT, NTL, MODE, false>(ring, in, in_stride, M, lw, lb, mask, mask_stride, bits, out,
                                          out_stride, tile, next_row, lo, hi, col0);
     }
     u = un;
@@ -963,7 +985,8 @@ static void launch_layer(const void* in, int in_stride, int64_t M, const int64_t
 template <int KT, int NTL, int NCG, int MODE>
 static void launch_layer_split(const void* in, int in_stride, int64_t M, const int64_t* Md,
                                const void* wpk, const float* bias, const void* mask,
-                               int mask_stride, void* out, int out_stride, hipStream_t st) {
+                               int mask_stride, void* out, int out_stride, uint32_t* bits,
+                               hipStream_t st) {
   // per column group: about two 32 x 32 output blocks per wave at least, at most one
   // block per CU over all groups (the weights fill the LDS); with two groups the group
   // size is a multiple of 8 (the block -> group map pairs blocks 8 apart)
@@ -976,7 +999,7 @@ static void launch_layer_split(const void* in, int in_stride, int64_t M, const i
   k_mlp_layer_split<KT, NTL, NCG, MODE><<<(unsigned)(b * NCG), kLayerThreads, 0, st>>>(
       reinterpret_cast<const float*>(in), in_stride, M, Md, reinterpret_cast<const bf16x8*>(wpk),
       bias, reinterpret_cast<const float*>(mask), mask_stride, reinterpret_cast<float*>(out),
-      out_stride);
+      out_stride, bits);
 }
 
 // Row blocks of the weight-gradient kernel: one per CU (each holds two row chunks in
@@ -1079,8 +1102,9 @@ int vx_mlp_pack_net(int prec, int n_hidden, const int64_t* w_ptrs, int k_in0, vo
 int vx_mlp_layer_fwd(int prec, int kt, int last, const void* in, int in_stride, int64_t M,
                      const void* wpk, const float* bias, void* out, int out_stride,
                      const float* w3, const float* b3, float* logits, const int64_t* M_dev,
-                     void* stream) {
+                     uint32_t* mask_bits, void* stream) {
   VX_CHECK_ARG(prec >= 0 && prec <= 2, "mlp_layer_fwd: prec must be 0, 1 or 2");
+  VX_CHECK_ARG(!mask_bits || prec == 2, "mlp_layer_fwd: mask bits need prec 2");
   const int bf16 = prec == 1;
   VX_CHECK_ARG(M >= 0, "mlp_layer_fwd: M < 0");
   VX_CHECK_ARG(kt == 2 || kt == 3 || kt == 6, "mlp_layer_fwd: input width must be 64, 96 or 192");
@@ -1088,7 +1112,7 @@ int vx_mlp_layer_fwd(int prec, int kt, int last, const void* in, int in_stride, 
   VX_CHECK_ARG(!last || (w3 && b3 && logits), "mlp_layer_fwd: last layer needs w3, b3, logits");
   hipStream_t st = (hipStream_t)stream;
 #define L(KT, BF) launch_layer<KT, 6, 0, BF>(in, in_stride, M, M_dev, wpk, bias, nullptr, 0, out, out_stride, st)
-#define LS(KT, NTL, NCG) launch_layer_split<KT, NTL, NCG, 0>(in, in_stride, M, M_dev, wpk, bias, nullptr, 0, out, out_stride, st)
+#define LS(KT, NTL, NCG) launch_layer_split<KT, NTL, NCG, 0>(in, in_stride, M, M_dev, wpk, bias, nullptr, 0, out, out_stride, mask_bits, st)
   if (prec == 2) {
     if (kt == 2) LS(2, 3, 2);
     else if (kt == 3) LS(3, 3, 2);
@@ -1127,15 +1151,18 @@ int vx_mlp_logits(int prec, const void* H, int hs, int64_t M, const float* w3, c
 
 int vx_mlp_layer_bwd(int prec, int nto, const float* dy, int dy_stride, int64_t M,
                      const void* wpk_t, const void* mask, int mask_stride, float* dx,
-                     int dx_stride, const int64_t* M_dev, void* stream) {
+                     int dx_stride, const int64_t* M_dev, const uint32_t* mask_bits,
+                     void* stream) {
   VX_CHECK_ARG(prec >= 0 && prec <= 2, "mlp_layer_bwd: prec must be 0, 1 or 2");
+  VX_CHECK_ARG(!mask_bits || (prec == 2 && mask && nto == 6),
+               "mlp_layer_bwd: mask bits need prec 2 and a masked 192-wide layer");
   const int bf16 = prec == 1;
   VX_CHECK_ARG(M >= 0, "mlp_layer_bwd: M < 0");
   VX_CHECK_ARG(nto == 2 || nto == 3 || nto == 6, "mlp_layer_bwd: output width must be 64, 96 or 192");
   if (M == 0) return VX_OK;
   hipStream_t st = (hipStream_t)stream;
 #define L(NTO, MODE, BF) launch_layer<6, NTO, MODE, BF>(dy, dy_stride, M, M_dev, wpk_t, nullptr, mask, mask_stride, dx, dx_stride, st)
-#define LS(NTL, NCG, MODE) launch_layer_split<6, NTL, NCG, MODE>(dy, dy_stride, M, M_dev, wpk_t, nullptr, mask, mask_stride, dx, dx_stride, st)
+#define LS(NTL, NCG, MODE) launch_layer_split<6, NTL, NCG, MODE>(dy, dy_stride, M, M_dev, wpk_t, nullptr, mask, mask_stride, dx, dx_stride, const_cast<uint32_t*>(mask_bits), st)
   if (prec == 2) {
     if (nto == 6) { if (mask) LS(3, 2, 2); else LS(3, 2, 3); }
     else if (nto == 3) LS(3, 1, 3);

@@ -322,8 +322,9 @@ class FusedFineTrainer:
             layer_ev = [] if self.layer_events is not None else None
             if n > 0:
                 Xr = mlp_hip.padded_rows(self._X, din, prec)
+                mbits = []
                 lg, hs = mlp_hip.forward_rows(Xr, Ws, bs, prec, self._packed, st, M_dev=m_dev,
-                                              timing=layer_ev)
+                                              timing=layer_ev, bits=mbits)
             else:
                 lg = torch.zeros((1, 3), dtype=torch.float32, device=self.dev)
             dlogit = torch.empty((max(rows, 1), 3), dtype=torch.float32, device=self.dev)
@@ -333,7 +334,7 @@ class FusedFineTrainer:
                  ptr(dlogit), ptr(gw), ptr(self.glast), ptr(self.S), ptr(self.loss_r), rows, st)
             if n > 0:
                 dX = mlp_hip.backward_rows(Xr, hs, Ws, prec, self._packed, dlogit, st,
-                                           self.mlp_grads, M_dev=m_dev)
+                                           self.mlp_grads, M_dev=m_dev, bits=mbits)
             else:
                 # no ray: every rgbnet gradient is zero
                 self.mlp_gflat.zero_()

@@ -78,12 +78,14 @@ class Packed:
         self.prec = prec
 
 
-def forward_rows(X, Ws, bs, prec, packed, st, M_dev=None, timing=None):
+def forward_rows(X, Ws, bs, prec, packed, st, M_dev=None, timing=None, bits=None):
     """logits [M, 3] and the hidden activations of X [M, 32*kt0] (padded rows in the
     activation element type).  M_dev: optional device int64 row count (at most M rows of
     the buffers are then used), so the launches need not wait for it on the host.
     timing: a list that receives (start, end, launches): events around the run of
-    192 x 192 layers (back to back on the stream) and their number."""
+    192 x 192 layers (back to back on the stream) and their number.  bits: a list that
+    receives each hidden layer's ReLU mask as bits [M, 6] int32 (split precision only;
+    backward_rows reads them instead of the activations' signs)."""
     M = X.shape[0]
     dev = X.device
     adt = act_dtype(prec)
@@ -97,8 +99,13 @@ def forward_rows(X, Ws, bs, prec, packed, st, M_dev=None, timing=None):
         if timing is not None and li == 1:
             e0 = torch.cuda.Event(enable_timing=True)
             e0.record()
+        mb = None
+        if bits is not None and prec == PREC_F32_SPLIT:
+            mb = torch.empty((M, WIDTH // 32), dtype=torch.int32, device=dev)
+            bits.append(mb)
         call("vx_mlp_layer_fwd", prec, kt, 0, ptr(h_in), h_in.shape[1], M,
-             ptr(packed.fwd[li]), ptr(bs[li]), ptr(H), WIDTH, None, None, None, ptr(M_dev), st)
+             ptr(packed.fwd[li]), ptr(bs[li]), ptr(H), WIDTH, None, None, None, ptr(M_dev),
+             ptr(mb), st)
         if timing is not None and li == nlin - 2 and li >= 1:
             e1 = torch.cuda.Event(enable_timing=True)
             e1.record()
@@ -111,10 +118,10 @@ def forward_rows(X, Ws, bs, prec, packed, st, M_dev=None, timing=None):
     return logits, hs
 
 
-def backward_rows(X, hs, Ws, prec, packed, g, st, grads, M_dev=None):
+def backward_rows(X, hs, Ws, prec, packed, g, st, grads, M_dev=None, bits=None):
     """Gradients of every weight and bias into the tensors of `grads` ([W0, b0, W1, ...],
-    overwritten) and the gradient of X [M, 32*kt0] (returned, float32); M_dev as for
-    forward_rows."""
+    overwritten) and the gradient of X [M, 32*kt0] (returned, float32); M_dev and bits
+    as filled by forward_rows."""
     M = X.shape[0]
     dev = X.device
     nlin = len(Ws)
@@ -137,12 +144,13 @@ def backward_rows(X, hs, Ws, prec, packed, g, st, grads, M_dev=None):
         if li > 0:
             dprev = torch.empty((M, WIDTH), dtype=torch.float32, device=dev)
             call("vx_mlp_layer_bwd", prec, kt, ptr(dH), WIDTH, M, ptr(packed.bwd[li]),
-                 ptr(h_prev), WIDTH, ptr(dprev), WIDTH, ptr(M_dev), st)
+                 ptr(h_prev), WIDTH, ptr(dprev), WIDTH, ptr(M_dev),
+                 ptr(bits[li - 1]) if bits else None, st)
             dH = dprev
         else:
             dX = torch.empty((M, 32 * kt), dtype=torch.float32, device=dev)
             call("vx_mlp_layer_bwd", prec, kt, ptr(dH), WIDTH, M, ptr(packed.bwd[0]), None,
-                 0, ptr(dX), 32 * kt, ptr(M_dev), st)
+                 0, ptr(dX), 32 * kt, ptr(M_dev), None, st)
     return dX
 
 
@@ -166,8 +174,10 @@ class _MlpFn(torch.autograd.Function):
         st = stream_of(x)
         X = padded_rows(x, Ws[0].shape[1], prec)
         packed = Packed(Ws, prec, st)
-        logits, hs = forward_rows(X, Ws, bs, prec, packed, st)
+        bits = []
+        logits, hs = forward_rows(X, Ws, bs, prec, packed, st, bits=bits)
         ctx.save_for_backward(X, *hs)
+        ctx.bits = bits
         ctx.Ws, ctx.bs, ctx.packed, ctx.prec, ctx.din = Ws, bs, packed, prec, x.shape[1]
         return logits
 
@@ -180,7 +190,7 @@ class _MlpFn(torch.autograd.Function):
         for W, b in zip(Ws, bs):
             grads += [torch.empty_like(W), torch.empty_like(b)]
         dX = backward_rows(X, hs, Ws, ctx.prec, ctx.packed, glogits.contiguous().float(),
-                           stream_of(X), grads)
+                           stream_of(X), grads, bits=ctx.bits)
         return (dX[:, :ctx.din], None, *grads)
 
 
