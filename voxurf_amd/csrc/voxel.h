@@ -394,10 +394,35 @@ __device__ __forceinline__ void sdf7(const T* __restrict__ grid, const GridDesc&
     shared = shared && d[(c + 1) % 3] == 0 && d[(c + 2) % 3] == 0 && (d[c] == 0 || d[c] == s);
   }
   T f[7];  // the seven lookups: centre, then -/+ along z, y, x of the zyx index
-  if (__all(shared)) {
+  // element offsets in 32 bits (the grid is below 2^31 elements; uniform)
+  const bool idx32 = (int64_t)(g.X - 1) * g.sx + (int64_t)(g.Y - 1) * g.sy +
+                         (int64_t)(g.Z - 1) * g.sz < ((int64_t)1 << 31);
+  if (idx32 && __all(shared)) {
+    // per axis, the clamped offsets and in-grid flags of lattice lines base-1 .. base+2
+    int ox[4], oy[4], oz[4];
+    unsigned vx = 0u, vy = 0u, vz = 0u;
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+      const int x = t[0].x0 + d - 1, y = t[0].y0 + d - 1, z = t[0].z0 + d - 1;
+      vx |= ((unsigned)x < (unsigned)g.X ? 1u : 0u) << d;
+      vy |= ((unsigned)y < (unsigned)g.Y ? 1u : 0u) << d;
+      vz |= ((unsigned)z < (unsigned)g.Z ? 1u : 0u) << d;
+      ox[d] = clamp_index(x, g.X) * (int)g.sx;
+      oy[d] = clamp_index(y, g.Y) * (int)g.sy;
+      oz[d] = clamp_index(z, g.Z) * (int)g.sz;
+    }
+    auto ld = [&](int dx, int dy, int dz) { return grid[ox[dx] + oy[dy] + oz[dz]]; };
+    auto ok = [&](int dx, int dy, int dz) {
+      return ((vx >> dx) & (vy >> dy) & (vz >> dz) & 1u) != 0u;
+    };
     T v0[8], vo[3][2][4];
-    unsigned in0, ino[3][2];
-    tri_load(grid, g, t[0], 0, v0, in0);
+    unsigned in0 = 0u, ino[3][2];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int dx = 1 + (k >> 2), dy = 1 + ((k >> 1) & 1), dz = 1 + (k & 1);
+      v0[k] = ld(dx, dy, dz);
+      if (ok(dx, dy, dz)) in0 |= 1u << k;
+    }
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -410,13 +435,11 @@ __device__ __forceinline__ void sdf7(const T* __restrict__ grid, const GridDesc&
           // the voxel beyond the centre cell along axis c; j holds the corner bits of
           // the other two axes (the lower-numbered axis first)
           int o[3];
-          o[c] = side ? 2 : -1;
-          o[lo] = j >> 1;
-          o[hi] = j & 1;
-          const int x = t[0].x0 + o[0], y = t[0].y0 + o[1], z = t[0].z0 + o[2];
-          if (in_grid<T>(x, y, z, g)) ino[c][side] |= 1u << j;
-          vo[c][side][j] = grid[clamp_index(x, g.X) * g.sx + clamp_index(y, g.Y) * g.sy +
-                                clamp_index(z, g.Z) * g.sz];
+          o[c] = side ? 3 : 0;
+          o[lo] = 1 + (j >> 1);
+          o[hi] = 1 + (j & 1);
+          vo[c][side][j] = ld(o[0], o[1], o[2]);
+          if (ok(o[0], o[1], o[2])) ino[c][side] |= 1u << j;
         }
       }
     f[0] = tri_sum(t[0], v0, in0);
