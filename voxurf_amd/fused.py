@@ -19,10 +19,12 @@ One step (run.py:600-659 for the fine model, lib/voxurf_fine.py:620-802):
   9. Adam (utils.Adam semantics) over sdf, k0 and rgbnet, clearing the grid gradients.
 
 Streams.  The k0 Adam pass (the largest HBM sweep of the step) runs on a second stream
-in two halves (vx_adam_step_split): the unflagged pages, whose gradient is zero this
-step, from the end of step 1 -- concurrently with steps 2-8 -- and the flagged pages
-after step 7.  The main stream waits for it only before the next step's gather reads
-k0, so it also overlaps the next forward.
+in two halves (vx_adam_step_split): the touched unflagged pages, whose gradient is zero
+this step, from the end of step 1 -- concurrently with steps 2-8 -- and the flagged
+pages after step 8, beside the SDF Adam pass.  The main stream waits for it only before
+the next step's gather reads k0, so it may also overlap the next forward.  The survivor
+count stays on the device (step 2): the colour side is queued on it and the host reads
+it back before the optimiser.
 
 Grids: sdf [1,1,R,R,R] contiguous; k0 [1,C,R,R,R] channels-last.  Gradient buffers
 persist across steps (cleared by the Adam pass instead of reallocated).
