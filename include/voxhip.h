@@ -356,13 +356,16 @@ int vx_segment_gather(int dtype, const void* gout, const int64_t* index, int64_t
  * xyz_max[3]}; samples it rejects are dropped like out-of-box ones
  * (lib/voxurf_fine.py:631-636).  k0_pages (may be NULL, then k0_geom may be too): set the
  * page flag (vx_adam_step) of every k0_grad page the feature scatter of the survivors
- * can write, as soon as they are known. */
+ * can write, as soon as they are known.  sdf_pages (may be NULL): the same for every
+ * SDF-gradient page the stencil scatter of vx_fused_fine_bwd can write (walk-range
+ * samples). */
 int vx_fused_fine_fwd(const float* sdf, const int64_t* sdf_geom, const float* box,
                       const float* params, const float* mc, const int64_t* mc_geom,
                       const float* mc_params, const float* rays_o, const float* rays_d,
                       const float* viewdirs, int64_t n_rays, int cap, void* s4, void* g4,
                       int64_t* cnt, float* last, int* stats, int count_all, int* overflow,
-                      const int64_t* k0_geom, uint32_t* k0_pages, void* stream);
+                      const int64_t* k0_geom, uint32_t* k0_pages, uint32_t* sdf_pages,
+                      void* stream);
 /* Keeps the survivors (weight > thres) of each ray's slots at rows
  * off_incl[r]-cnt[r] .. off_incl[r]-1 and builds the rgbnet input rows
  * X [M, xstride] = [k0 | normal | view PE | 0] (lib/voxurf_fine.py:678-716; xstride a

@@ -161,13 +161,16 @@ def test_split_k0_update_over_consecutive_steps(dev, setup):
 def test_touched_page_skip_over_changing_batches(dev, setup):
     """Three steps on different ray batches: pages touched by an earlier step but not
     flagged now are swept with their decaying moments, pages never touched are skipped
-    (their update is the identity); after every step k0 and its moments equal utils.Adam
-    applied in order to the whole grid."""
+    (their update is the identity); after every step k0 and the SDF grid, with their
+    moments, equal utils.Adam applied in order to the whole grid."""
     st, _, _, _ = setup
     t = FusedFineTrainer(CFG, st["sdf"], st["k0"], st["mlp"], device=dev)
     p = t.k0.detach().cpu().double()
     m = torch.zeros_like(p)
     vv = torch.zeros_like(p)
+    ps = t.sdf.detach().cpu().double()
+    ms = torch.zeros_like(ps)
+    vs = torch.zeros_like(ps)
     for i in range(3):
         o, d, v = (x.to(dev) for x in scene.rays(CFG.n_rays, seed=50 + i))
         t.step(o, d, v, scene.targets(CFG.n_rays, seed=50 + i).to(dev), profile=True)
@@ -176,11 +179,20 @@ def test_touched_page_skip_over_changing_batches(dev, setup):
         close(t.k0, p, 1e-5, 1e-7, f"k0 after step {i + 1}")
         close(t.adam["k0"][0], m, 1e-5, 1e-7, f"k0 exp_avg after step {i + 1}")
         close(t.adam["k0"][1], vv, 1e-5, 1e-7, f"k0 exp_avg_sq after step {i + 1}")
+        # the SDF grid's two-phase pass (page flags from the forward) likewise
+        gs = t.grads_snapshot["sdf"].cpu().double()
+        tr.adam_step(ps, gs, ms, vs, i + 1, CFG.lr_sdf, 0.9, 0.99, 1e-8)
+        close(t.sdf, ps, 1e-5, 1e-7, f"sdf after step {i + 1}")
+        close(t.adam["sdf"][0], ms, 1e-5, 1e-7, f"sdf exp_avg after step {i + 1}")
+        close(t.adam["sdf"][1], vs, 1e-5, 1e-7, f"sdf exp_avg_sq after step {i + 1}")
         if i > 0:
             assert t.last_swept_pages > 0   # touched earlier, not flagged now
         p = t.k0.detach().cpu().double()
         m = t.adam["k0"][0].detach().cpu().double()
         vv = t.adam["k0"][1].detach().cpu().double()
+        ps = t.sdf.detach().cpu().double()
+        ms = t.adam["sdf"][0].detach().cpu().double()
+        vs = t.adam["sdf"][1].detach().cpu().double()
     assert 0 < t.touched_pages() < t.n_pages
     # untouched pages: parameters exactly as initialised, moments exactly zero
     untouched = (t._touched == 0).cpu()

@@ -32,6 +32,7 @@ struct ChunkState {
   NeusFwd<float> nf;
   float alpha, myT, w;
   bool inb, act, inrange, surv;
+  bool shared;  // the sample's 7-point cells lie in the 32 voxels around its cell (sdf7)
 };
 
 __device__ __forceinline__ void run_chunk(const FusedGeom& G, const float* start, const float* dir,
@@ -44,10 +45,11 @@ __device__ __forceinline__ void run_chunk(const FusedGeom& G, const float* start
   cs.inb = valid && !out;
   if (G.mc && cs.inb) cs.inb = maskcache_pass(G, cs.q);
   cs.alpha = 0.f;
+  cs.shared = true;
   cs.sdf = 0.f;
   cs.grad[0] = cs.grad[1] = cs.grad[2] = 0.f;
   if (cs.inb) {
-    sdf7<float>(G.sdf, G.gs, cs.q, G.mn, G.mx, G.voxel_size, cs.sdf, cs.grad, nullptr);
+    sdf7<float>(G.sdf, G.gs, cs.q, G.mn, G.mx, G.voxel_size, cs.sdf, cs.grad, nullptr, &cs.shared);
     cs.nf = neus_fwd<float>(vd, cs.grad, cs.sdf, G.stepdist, G.inv_s);
     cs.alpha = clip01(cs.nf.a);
   }
@@ -139,7 +141,7 @@ __global__ void __launch_bounds__(256, 3) k_fused_fwd(
     const float* __restrict__ viewdirs, int64_t n_rays, int cap, float4* __restrict__ s4,
     float4* __restrict__ g4, int64_t* __restrict__ cnt_out, float* __restrict__ last_out,
     int* __restrict__ stats, int count_all, int* __restrict__ overflow,
-    uint32_t* __restrict__ k0_pages) {
+    uint32_t* __restrict__ k0_pages, uint32_t* __restrict__ sdf_pages) {
   const int lane = threadIdx.x & 63;
   const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
   for (int64_t r = blockIdx.x * (int64_t)(blockDim.x >> 6) + (threadIdx.x >> 6); r < n_rays;
@@ -166,6 +168,9 @@ __global__ void __launch_bounds__(256, 3) k_fused_fwd(
         }
       }
       m1 += __popcll(im);
+      // every SDF-gradient page the backward's stencil scatter of this walk-range sample
+      // can write
+      if (sdf_pages && cs.inrange) sdf7_mark_pages<float>(sdf_pages, G.gs, cs.q, G.mn, G.mx, cs.shared);
       cnt += __popcll(__ballot(cs.surv));
       if (k0_pages && cs.surv) {
         // every k0_grad page the feature scatter of this survivor can write
@@ -622,7 +627,8 @@ int vx_fused_fine_fwd(const float* sdf, const int64_t* sdf_geom, const float* bo
                       const float* mc_params, const float* rays_o, const float* rays_d,
                       const float* viewdirs, int64_t n_rays, int cap, void* s4, void* g4,
                       int64_t* cnt, float* last, int* stats, int count_all, int* overflow,
-                      const int64_t* k0_geom, uint32_t* k0_pages, void* stream) {
+                      const int64_t* k0_geom, uint32_t* k0_pages, uint32_t* sdf_pages,
+                      void* stream) {
   FusedGeom G;
   VX_CHECK_ARG(!k0_pages || k0_geom, "fused_fine_fwd: k0_pages needs k0_geom");
   if (int st = fused_geom(G, sdf, sdf_geom, nullptr, k0_geom, box, params)) return st;
@@ -631,7 +637,7 @@ int vx_fused_fine_fwd(const float* sdf, const int64_t* sdf_geom, const float* bo
   if (n_rays == 0) return VX_OK;
   k_fused_fwd<<<grid_for(n_rays, 4, 1 << 16), 256, 0, (hipStream_t)stream>>>(
       G, rays_o, rays_d, viewdirs, n_rays, cap, (float4*)s4, (float4*)g4, cnt, last, stats,
-      count_all, overflow, k0_pages);
+      count_all, overflow, k0_pages, sdf_pages);
   return launch_status("fused_fine_fwd");
 }
 

@@ -363,7 +363,7 @@ __device__ __forceinline__ void displaced_cell(const GridDesc& g, const T ind[3]
 template <typename T>
 __device__ __forceinline__ void sdf7(const T* __restrict__ grid, const GridDesc& g, const T q[3],
                                      const T* mn, const T* mx, T voxel_size, T& sdf, T grad[3],
-                                     T* feat) {
+                                     T* feat, bool* shared_out = nullptr) {
 #pragma clang fp contract(off)
   // the seven cells first, then all corner loads, then the sums: one memory latency per
   // sample instead of seven
@@ -393,6 +393,7 @@ __device__ __forceinline__ void sdf7(const T* __restrict__ grid, const GridDesc&
     dd[i - 1] = d[c];
     shared = shared && d[(c + 1) % 3] == 0 && d[(c + 2) % 3] == 0 && (d[c] == 0 || d[c] == s);
   }
+  if (shared_out) *shared_out = shared;
   T f[7];  // the seven lookups: centre, then -/+ along z, y, x of the zyx index
   // element offsets in 32 bits (the grid is below 2^31 elements; uniform)
   const bool idx32 = (int64_t)(g.X - 1) * g.sx + (int64_t)(g.Y - 1) * g.sy +
@@ -491,6 +492,51 @@ __device__ __forceinline__ void sdf7(const T* __restrict__ grid, const GridDesc&
       feat[2 * (2 - a)] = fm;
       feat[2 * (2 - a) + 1] = fp;
     }
+  }
+}
+
+// Set the page flag (pages[element >> kPageShift]) of every voxel the 7-point stencil of
+// q reaches, i.e. every voxel sdf7_scatter / sdf7_terms can add to: the 32 voxels around
+// the centre cell when every displaced cell is the shifted or clamped centre cell
+// (`shared`, as sdf7 reports it), else each cell's own 8 corners besides.
+template <typename T>
+__device__ __forceinline__ void sdf7_mark_pages(uint32_t* __restrict__ pages, const GridDesc& g,
+                                                const T q[3], const T* mn, const T* mx,
+                                                bool shared) {
+#pragma clang fp contract(off)
+  Tri<T> t0;
+  t0.init(world_to_index(q[0], mn[0], mx[0] - mn[0], g.X),
+          world_to_index(q[1], mn[1], mx[1] - mn[1], g.Y),
+          world_to_index(q[2], mn[2], mx[2] - mn[2], g.Z));
+  auto mark = [&](int x, int y, int z) {
+    if (in_grid<T>(x, y, z, g)) pages[(x * g.sx + y * g.sy + z * g.sz) >> kPageShift] = 1u;
+  };
+#pragma unroll
+  for (int k = 0; k < 8; ++k) mark(t0.x0 + (k >> 2), t0.y0 + ((k >> 1) & 1), t0.z0 + (k & 1));
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const int lo = (c + 1) % 3 < (c + 2) % 3 ? (c + 1) % 3 : (c + 2) % 3;
+    const int hi = (c + 1) % 3 < (c + 2) % 3 ? (c + 2) % 3 : (c + 1) % 3;
+#pragma unroll
+    for (int sd = 0; sd < 2; ++sd)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        int v[3] = {t0.x0, t0.y0, t0.z0};
+        v[c] += sd ? 2 : -1;
+        v[lo] += j >> 1;
+        v[hi] += j & 1;
+        mark(v[0], v[1], v[2]);
+      }
+  }
+  if (!shared) {
+    T ind[3];
+    zyx_index(q, mn, mx, g, ind);
+    for (int a = 0; a < 3; ++a)
+      for (int side = 0; side < 2; ++side) {
+        Tri<T> td;
+        displaced_cell(g, ind, a, side ? (T)1 : (T)-1, td);
+        for (int k = 0; k < 8; ++k) mark(td.x0 + (k >> 2), td.y0 + ((k >> 1) & 1), td.z0 + (k & 1));
+      }
   }
 }
 

@@ -91,8 +91,12 @@ class FusedFineTrainer:
         self._touched = torch.zeros(n_pages, dtype=torch.int32, device=self.dev)
         # the same for the SDF grid, whose one-pass update skips untouched pages with an
         # all-zero gradient (vx_adam_step_touched)
-        self._sdf_touched = torch.zeros((self.sdf.numel() + page - 1) // page, dtype=torch.int32,
-                                        device=self.dev)
+        n_sdf_pages = (self.sdf.numel() + page - 1) // page
+        self._sdf_touched = torch.zeros(n_sdf_pages, dtype=torch.int32, device=self.dev)
+        # page flags, count and list of the SDF gradient (set by the forward for the
+        # pages the SDF backward can write), for the two-phase SDF Adam pass
+        self._sdf_pages = torch.zeros(2 * n_sdf_pages + 1, dtype=torch.int32, device=self.dev)
+        self.n_sdf_pages = n_sdf_pages
         self.adam = {
             "sdf": [torch.zeros_like(self.sdf), torch.zeros_like(self.sdf), 0, cfg.lr_sdf],
             "k0": [torch.zeros_like(self._k0), torch.zeros_like(self._k0), 0, cfg.lr_k0],
@@ -208,7 +212,8 @@ class FusedFineTrainer:
                  float(lr), 0.9, 0.99, 1e-8, 1, ptr(pages), st)
         else:
             call("vx_adam_step_split", phase, ptr(p), ptr(g), ptr(m), ptr(v), None, p.numel(),
-                 step, float(lr), 0.9, 0.99, 1e-8, ptr(pages), ptr(self._touched), st)
+                 step, float(lr), 0.9, 0.99, 1e-8, ptr(pages),
+                 ptr(self._touched if touched is None else touched), st)
         if timed:
             e1.record()
             self.adam_events.append((e0, e1))
@@ -218,7 +223,7 @@ class FusedFineTrainer:
         """Steps 1-3: survivors and rgbnet input rows; returns M (survivor count)."""
         return self._forward(rays_o, rays_d, viewdirs, None, None)
 
-    def _forward(self, rays_o, rays_d, viewdirs, pages, k0_step, sync=True):
+    def _forward(self, rays_o, rays_d, viewdirs, pages, k0_step, sync=True, sdf_step=None):
         n = rays_o.shape[0]
         self._ensure(n)
         st = stream_of(self.sdf)
@@ -227,7 +232,8 @@ class FusedFineTrainer:
              self.params.data_ptr(), *self._mc_args(), ptr(rays_o), ptr(rays_d), ptr(viewdirs), n, self.cap,
              ptr(self.s4), ptr(self.g4), ptr(self.cnt), ptr(self.last), ptr(self.stats),
              int(self.count_all), ptr(self.overflow),
-             self.k0_geom.data_ptr() if pages is not None else None, ptr(pages), st)
+             self.k0_geom.data_ptr() if pages is not None else None, ptr(pages),
+             ptr(self._sdf_pages) if sdf_step is not None else None, st)
         if pages is not None and self.overlap_adam:
             # The forward has flagged every k0 page the feature scatter can write; the
             # other pages have an all-zero gradient this step, so their Adam update (most
@@ -240,6 +246,14 @@ class FusedFineTrainer:
                 self.side.wait_event(flags_ready)
                 self._adam("k0", self._k0, self.k0_grad, pages=pages, phase=1, step=k0_step,
                            timed=True)
+                if sdf_step is not None:
+                    # the same for the SDF grid: pages no walk-range stencil reaches have a
+                    # zero gradient this step, and the SDF grid is not read again before
+                    # the next forward (the backward works from the stored slots)
+                    self._adam("sdf", self.sdf, self.sdf_grad, pages=self._sdf_pages, phase=1,
+                               step=sdf_step, touched=self._sdf_touched)
+                    self._sdf_phase1 = torch.cuda.Event()
+                    self._sdf_phase1.record(self.side)
         call("vx_inclusive_scan_i64", ptr(self.cnt), ptr(self.off), n, ptr(self.ws), st)
         if n > 0:
             # the survivor count, on its way to the host while the rest is queued
@@ -312,7 +326,15 @@ class FusedFineTrainer:
         Ws = [l.weight.detach() for l in lin]
         bs = [l.bias.detach() for l in lin]
         prec = self.mlp_prec
-        self._forward(rays_o, rays_d, viewdirs, pages, k0_step, sync=False)
+        # The SDF Adam pass in two phases like k0's (pages without a gradient early, on the
+        # second stream), except on regulariser steps, whose dense terms add to every
+        # page's gradient and read the grid after the forward.
+        tv_step = tv_active(self.tv, k0_step)
+        sdf_step = None
+        if self.overlap_adam and not tv_step:
+            sdf_step = self.adam["sdf"][2] + 1
+            self.adam["sdf"][2] = sdf_step
+        self._forward(rays_o, rays_d, viewdirs, pages, k0_step, sync=False, sdf_step=sdf_step)
         # The colour side is queued with the survivor count still on the device (off[n-1]):
         # every kernel reads it there and processes min(count, capacity) rows, so the GPU
         # runs on while the host waits for the count.  Should the count exceed the row
@@ -353,16 +375,23 @@ class FusedFineTrainer:
             self._grow_rows(m)
             self._gather(n, viewdirs)
         self._set_rows(m)
+        # mean squared error over the batch's N x 3 colour components (run.py:604)
+        loss = torch.empty(1, dtype=torch.float32, device=self.dev)
+        call("vx_sum_f32", ptr(self.loss_r), n, 1.0 / (3.0 * n), ptr(loss), st)
         if layer_ev:
             self.layer_events += [(e0, e1, k, m) for e0, e1, k in layer_ev]
         if profile:
             # snapshot before any optimiser pass clears a gradient
             k0_grad_snapshot = self.k0_grad.clone()
+            mlp_grad_snapshot = [g.clone() for g in self.mlp_grads]
             flags = pages[:self.n_pages] != 0
             self.last_live_pages = int(flags.sum())
             # pages phase 1 swept this step: touched before it and not flagged (phase 2
             # of this step has not run yet, so touched is still the state phase 1 read)
             self.last_swept_pages = int(((self._touched != 0) & ~flags).sum())
+        # the rgbnet gradients are final: its Adam pass (small) goes before the SDF backward,
+        # ahead of the k0 sweep on the second stream
+        self._adam("mlp", self.mlp_flat, self.mlp_gflat)
         call("vx_fused_fine_bwd", ptr(self.sdf), self.sdf_geom.data_ptr(), self.box.data_ptr(),
              self.params.data_ptr(), ptr(viewdirs), n, self.cap, ptr(self.s4), ptr(self.g4),
              ptr(self.stats), ptr(self.cnt), ptr(self.off), ptr(self.last), ptr(gw),
@@ -385,15 +414,17 @@ class FusedFineTrainer:
             # the one-pass update does not record touched pages: count every page as
             # touched from now on (skipping is then off, which is always exact)
             self._touched.fill_(1)
-        tv_loss = self._regularise(n, k0_step) if tv_active(self.tv, k0_step) else None
+        tv_loss = self._regularise(n, k0_step) if tv_step else None
         if profile:
             self.grads_snapshot = dict(sdf=self.sdf_grad.clone(), k0=k0_grad_snapshot,
-                                       mlp=[g.clone() for g in self.mlp_grads])
-        self._adam("sdf", self.sdf, self.sdf_grad, touched=self._sdf_touched)
-        self._adam("mlp", self.mlp_flat, self.mlp_gflat)
-        # mean squared error over the batch's N x 3 colour components (run.py:604)
-        loss = torch.empty(1, dtype=torch.float32, device=self.dev)
-        call("vx_sum_f32", ptr(self.loss_r), n, 1.0 / (3.0 * n), ptr(loss), st)
+                                       mlp=mlp_grad_snapshot)
+        if sdf_step is not None:
+            # the flagged SDF pages, after phase 1 has listed them
+            main.wait_event(self._sdf_phase1)
+            self._adam("sdf", self.sdf, self.sdf_grad, pages=self._sdf_pages, phase=2,
+                       step=sdf_step, touched=self._sdf_touched)
+        else:
+            self._adam("sdf", self.sdf, self.sdf_grad, touched=self._sdf_touched)
         if tv_loss is not None:
             loss += tv_loss
         if profile:
@@ -438,6 +469,7 @@ class FusedFineTrainer:
         those of a long one, whose rays have visited the scene widely."""
         self.k0  # waits for a pending k0 update
         flags = torch.zeros(2 * self.n_pages + 1, dtype=torch.int32, device=self.dev)
+        sdf_flags = torch.zeros_like(self._sdf_touched)
         for rays_o, rays_d, viewdirs in batches:
             n = rays_o.shape[0]
             self._ensure(n)
@@ -445,21 +477,9 @@ class FusedFineTrainer:
                  self.params.data_ptr(), *self._mc_args(), ptr(rays_o), ptr(rays_d),
                  ptr(viewdirs), n, self.cap, ptr(self.s4), ptr(self.g4), ptr(self.cnt),
                  ptr(self.last), ptr(self.stats), 0, ptr(self.overflow), self.k0_geom.data_ptr(),
-                 ptr(flags), stream_of(self.sdf))
-            # SDF pages: the 7-point stencils (one voxel around the sample's cell) of the
-            # walk-range samples, the only samples with an SDF gradient
-            valid = (torch.arange(self.cap, device=self.dev)[None, :] <
-                     self.stats[:n, 1:2].long()).reshape(-1)
-            q = self.s4[:n * self.cap][valid, :3]
-            R = torch.tensor(self.sdf.shape[-3:], device=self.dev)
-            idx = ((q + 1.0) / 2.0 * (R - 1).float()).floor().long()
-            off = torch.stack(torch.meshgrid(*[torch.arange(-1, 3, device=self.dev)] * 3,
-                                             indexing="ij"), -1).reshape(-1, 3)
-            cells = (idx[:, None, :] + off[None]).reshape(-1, 3)
-            cells = torch.minimum(torch.maximum(cells, torch.zeros_like(R)), R - 1)
-            lin = (cells[:, 0] * R[1] + cells[:, 1]) * R[2] + cells[:, 2]
-            self._sdf_touched[torch.unique(lin // self.page)] = 1
+                 ptr(flags), ptr(sdf_flags), stream_of(self.sdf))
         self._touched |= (flags[:self.n_pages] != 0).to(torch.int32)
+        self._sdf_touched |= (sdf_flags != 0).to(torch.int32)
 
     def touched_pages(self):
         """Number of k0 pages counted as touched."""
