@@ -392,15 +392,10 @@ class FusedFineTrainer:
         # the rgbnet gradients are final: its Adam pass (small) goes before the SDF backward,
         # ahead of the k0 sweep on the second stream
         self._adam("mlp", self.mlp_flat, self.mlp_gflat)
-        call("vx_fused_fine_bwd", ptr(self.sdf), self.sdf_geom.data_ptr(), self.box.data_ptr(),
-             self.params.data_ptr(), ptr(viewdirs), n, self.cap, ptr(self.s4), ptr(self.g4),
-             ptr(self.stats), ptr(self.cnt), ptr(self.off), ptr(self.last), ptr(gw),
-             ptr(self.glast), ptr(self.S), ptr(dgf), ptr(self.sdf_grad), rows, st)
         if self.overlap_adam:
-            # The flagged pages, from the end of the SDF backward: the sweep (HBM-bound)
-            # then runs beside the SDF Adam pass and the next step's forward, which does
-            # not read k0 (the next gather waits for it), instead of beside the SDF
-            # backward's scattered atomics.
+            # The flagged pages, once the feature scatter has finished: the sweep
+            # (HBM-bound) runs beside the SDF backward (scattered atomics) and the SDF Adam
+            # pass, and ends before the next step's gather reads k0.
             grads_ready = torch.cuda.Event()
             grads_ready.record(main)
             with torch.cuda.stream(self.side):
@@ -414,6 +409,10 @@ class FusedFineTrainer:
             # the one-pass update does not record touched pages: count every page as
             # touched from now on (skipping is then off, which is always exact)
             self._touched.fill_(1)
+        call("vx_fused_fine_bwd", ptr(self.sdf), self.sdf_geom.data_ptr(), self.box.data_ptr(),
+             self.params.data_ptr(), ptr(viewdirs), n, self.cap, ptr(self.s4), ptr(self.g4),
+             ptr(self.stats), ptr(self.cnt), ptr(self.off), ptr(self.last), ptr(gw),
+             ptr(self.glast), ptr(self.S), ptr(dgf), ptr(self.sdf_grad), rows, st)
         tv_loss = self._regularise(n, k0_step) if tv_step else None
         if profile:
             self.grads_snapshot = dict(sdf=self.sdf_grad.clone(), k0=k0_grad_snapshot,
