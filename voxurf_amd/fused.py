@@ -20,8 +20,9 @@ One step (run.py:600-659 for the fine model, lib/voxurf_fine.py:620-802):
 
 Streams.  The k0 Adam pass (the largest HBM sweep of the step) runs on a second stream
 in two halves (vx_adam_step_split): the touched unflagged pages, whose gradient is zero
-this step, from the end of step 1 -- concurrently with steps 2-8 -- and the flagged
-pages after step 8, beside the SDF Adam pass.  The main stream waits for it only before
+this step, from the end of step 1 -- concurrently with steps 2-7 -- and the flagged
+pages after step 7, beside the SDF backward.  The SDF grid's pass is split the same way
+(phase 1 on the second stream, phase 2 after step 8 on the main stream).  The main stream waits for it only before
 the next step's gather reads k0, so it may also overlap the next forward.  The survivor
 count stays on the device (step 2): the colour side is queued on it and the host reads
 it back before the optimiser.
