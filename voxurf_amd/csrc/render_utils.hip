@@ -177,30 +177,41 @@ __global__ void k_scan_add(int64_t* __restrict__ out, int64_t n,
   if (i < n) out[i] += tile_off[i / kScanTile];
 }
 
-// Single block for short inputs: the tiles in order with a running carry (one launch
-// instead of three).
-__global__ void __launch_bounds__(kScanThreads) k_scan_small(const int64_t* __restrict__ in,
-                                                             int64_t* __restrict__ out, int64_t n,
-                                                             int64_t* __restrict__ total_out) {
-  __shared__ int64_t lds[kScanThreads / 64];
-  int64_t carry = 0;
-  for (int64_t t0 = 0; t0 < n; t0 += kScanTile) {
-    const int64_t base = t0 + threadIdx.x * 4;
-    int64_t v[4];
-    int64_t s = 0;
-    for (int i = 0; i < 4; ++i) {
-      v[i] = (base + i < n) ? in[base + i] : 0;
-      s += v[i];
-    }
-    int64_t total;
-    int64_t off = carry + block_excl_scan(s, lds, total);
-    for (int i = 0; i < 4; ++i) {
-      off += v[i];
-      if (base + i < n) out[base + i] = off;
-    }
-    carry += total;
+// Single block of 1024 threads for inputs up to 16 K elements: each thread scans a run of
+// up to 16 consecutive elements (all loads in flight together), one block-wide scan of
+// the run totals.
+constexpr int kScanOneThreads = 1024, kScanOnePer = 16;
+__global__ void __launch_bounds__(kScanOneThreads) k_scan_one(const int64_t* __restrict__ in,
+                                                              int64_t* __restrict__ out,
+                                                              int64_t n,
+                                                              int64_t* __restrict__ total_out) {
+  __shared__ int64_t lds[kScanOneThreads / 64];
+  const int per = (int)((n + kScanOneThreads - 1) / kScanOneThreads);  // <= kScanOnePer
+  const int64_t base = (int64_t)threadIdx.x * per;
+  int64_t v[kScanOnePer];
+  int64_t s = 0;
+#pragma unroll
+  for (int i = 0; i < kScanOnePer; ++i) {
+    v[i] = (i < per && base + i < n) ? in[base + i] : 0;
+    s += v[i];
   }
-  if (threadIdx.x == 0 && total_out) *total_out = carry;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t inc = wave_incl_scan_i64(s);
+  if (lane == 63) lds[w] = inc;
+  __syncthreads();
+  int64_t off = 0, total = 0;
+#pragma unroll
+  for (int i = 0; i < kScanOneThreads / 64; ++i) {
+    if (i < w) off += lds[i];
+    total += lds[i];
+  }
+  off += inc - s;
+#pragma unroll
+  for (int i = 0; i < kScanOnePer; ++i) {
+    off += v[i];
+    if (i < per && base + i < n) out[base + i] = off;
+  }
+  if (threadIdx.x == 0 && total_out) *total_out = total;
 }
 
 template <typename T>
@@ -213,8 +224,8 @@ static int scan_i64(const int64_t* in, int64_t* out, int64_t n, int64_t* ws, int
     return launch_status("scan");
   }
   const int64_t ntiles = cdiv(n, kScanTile);
-  if (ntiles <= 16) {
-    k_scan_small<<<1, kScanThreads, 0, st>>>(in, out, n, total);
+  if (n <= (int64_t)kScanOneThreads * kScanOnePer) {
+    k_scan_one<<<1, kScanOneThreads, 0, st>>>(in, out, n, total);
     return launch_status("scan");
   }
   k_scan_tiles<<<(unsigned)ntiles, kScanThreads, 0, st>>>(in, out, n, ws);
