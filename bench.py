@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serial-adam", action="store_true",
                     help="run the k0 Adam pass in order on the main stream (per-kernel timing)")
+    ap.add_argument("--serial-wgrad", action="store_true",
+                    help="run rgbnet's weight gradients and Adam pass in order on the main stream")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--mode", choices=["train", "coarse", "recompute", "precision", "serve",
                                        "kernels", "next_rows"],
@@ -125,6 +127,7 @@ def main():
     if args.path == "fused":
         trainer = FusedFineTrainer(cfg, sdf, k0, mlp, device=dev, mlp_dtype=mlp_dtype)
         trainer.overlap_adam = not args.serial_adam
+        trainer.overlap_wgrad = not args.serial_wgrad
         trainer.rgbnet.f32_split = args.mlp != "fp32-mfma"
     else:
         model = VoxurfFine(cfg, sdf, k0, mlp, mlp_dtype=mlp_dtype).to(dev)

@@ -23,7 +23,12 @@ in two halves (vx_adam_step_split): the touched unflagged pages, whose gradient 
 this step, from the end of step 1 -- concurrently with steps 2-7 -- and the flagged
 pages after step 7, beside the SDF backward.  The SDF grid's pass is split the same way
 (phase 1 on the second stream, phase 2 after step 8 on the main stream).  The main stream waits for it only before
-the next step's gather reads k0, so it may also overlap the next forward.  The survivor
+the next step's gather reads k0, so it may also overlap the next forward.  rgbnet's
+hidden-layer weight gradients (MFMA-bound) and its Adam pass run on a third stream,
+queued once step 6's data gradients are done, beside steps 7-8 (atomic- and HBM-bound)
+and on into the next step's forward: that step's survivor scan waits only for the input
+layer's pass (the first, the one that reads the rows and the count), its weight packing
+for the Adam pass.  The survivor
 count stays on the device (step 2): the colour side is queued on it and the host reads
 it back before the optimiser.
 
@@ -102,15 +107,17 @@ class FusedFineTrainer:
             "sdf": [torch.zeros_like(self.sdf), torch.zeros_like(self.sdf), 0, cfg.lr_sdf],
             "k0": [torch.zeros_like(self._k0), torch.zeros_like(self._k0), 0, cfg.lr_k0],
         }
-        self.rgbnet = RgbNet(cfg.mlp_in, cfg.mlp_width, cfg.mlp_depth, params=mlp_params,
-                             compute_dtype=mlp_dtype).to(self.dev)
-        mlp_hip.check_shapes(self.rgbnet.linears())
+        self._rgbnet = RgbNet(cfg.mlp_in, cfg.mlp_width, cfg.mlp_depth, params=mlp_params,
+                              compute_dtype=mlp_dtype).to(self.dev)
+        self._mlp_done = None        # event: the last step's rgbnet update has finished
+        self._inputs_free = None     # event: its weight gradients no longer read X or off
+        mlp_hip.check_shapes(self._rgbnet.linears())
         if mlp_dtype not in (torch.float32, torch.bfloat16):
             raise RuntimeError("the fused step runs rgbnet in float32 or bfloat16")
         # rgbnet parameters (W0, b0, W1, ...) live in one flat buffer, and so do their
         # gradients and Adam moments: one Adam launch per step for the whole network
         # (every rgbnet parameter shares one learning rate and step count).
-        params = list(self.rgbnet.parameters())
+        params = list(self._rgbnet.parameters())
         total = sum(p.numel() for p in params)
         self.mlp_flat = torch.empty(total, dtype=torch.float32, device=self.dev)
         self.mlp_gflat = torch.zeros(total, dtype=torch.float32, device=self.dev)
@@ -139,6 +146,11 @@ class FusedFineTrainer:
         # True: the k0 Adam pass runs on the second stream in two phases (see step());
         # False: in one piece, in order on the main stream
         self.overlap_adam = True
+        # True: rgbnet's hidden-layer weight gradients and its Adam pass run on a third
+        # stream, queued behind the data gradients, beside the feature scatter and the
+        # SDF backward; False: in order on the main stream
+        self.wside = torch.cuda.Stream(self.dev)
+        self.overlap_wgrad = True
         # a list: (start, end) events of the k0 Adam sweep (phase 1, or the whole pass when
         # not overlapped) of every step are appended to it
         self.adam_events = None
@@ -158,7 +170,7 @@ class FusedFineTrainer:
     @property
     def mlp_prec(self):
         """Precision of the rgbnet kernels (voxurf_amd.mlp_hip.prec_of)."""
-        return mlp_hip.prec_of(self.rgbnet.compute_dtype, self.rgbnet.f32_split)
+        return mlp_hip.prec_of(self._rgbnet.compute_dtype, self._rgbnet.f32_split)
 
     @property
     def k0(self):
@@ -167,6 +179,20 @@ class FusedFineTrainer:
         if self._k0_done is not None:
             torch.cuda.current_stream(self.dev).wait_event(self._k0_done)
         return self._k0
+
+    @property
+    def rgbnet(self):
+        """The colour MLP, once its pending update (weight gradients and Adam on the
+        third stream) has finished (for reads on torch's current stream)."""
+        if self._mlp_done is not None:
+            torch.cuda.current_stream(self.dev).wait_event(self._mlp_done)
+        return self._rgbnet
+
+    def wait_pending(self):
+        """Make torch's current stream wait for every update still running on the other
+        streams (k0 and rgbnet)."""
+        self.k0
+        self.rgbnet
 
     def _mc_args(self):
         """(density ptr, geom ptr, params ptr) of the optional MaskCache, or NULLs."""
@@ -255,6 +281,11 @@ class FusedFineTrainer:
                                step=sdf_step, touched=self._sdf_touched)
                     self._sdf_phase1 = torch.cuda.Event()
                     self._sdf_phase1.record(self.side)
+        # the previous step's input-layer weight gradient reads the rows (X) the gather
+        # rewrites, after a copy of the survivor count (off) the scan rewrites
+        if self._inputs_free is not None:
+            main.wait_event(self._inputs_free)
+            self._inputs_free = None
         call("vx_inclusive_scan_i64", ptr(self.cnt), ptr(self.off), n, ptr(self.ws), st)
         if n > 0:
             # the survivor count, on its way to the host while the rest is queued
@@ -270,7 +301,11 @@ class FusedFineTrainer:
         # larger buffers.
         self._gather(n, viewdirs)
         if k0_step is not None:
-            lin = self.rgbnet.linears()
+            # the previous step's rgbnet Adam pass writes the weights packed here
+            if self._mlp_done is not None:
+                main.wait_event(self._mlp_done)
+                self._mlp_done = None
+            lin = self._rgbnet.linears()
             self._packed = mlp_hip.Packed([l.weight.detach() for l in lin], self.mlp_prec, st)
         if not sync:
             return None
@@ -323,7 +358,7 @@ class FusedFineTrainer:
         st = stream_of(self.sdf)
         main = torch.cuda.current_stream(self.dev)
         din = cfg.mlp_in
-        lin = self.rgbnet.linears()
+        lin = self._rgbnet.linears()
         Ws = [l.weight.detach() for l in lin]
         bs = [l.bias.detach() for l in lin]
         prec = self.mlp_prec
@@ -358,8 +393,12 @@ class FusedFineTrainer:
                  ptr(self._P4), ptr(self.last), ptr(target), float(cfg.bg), ptr(self.rgb),
                  ptr(dlogit), ptr(gw), ptr(self.glast), ptr(self.S), ptr(self.loss_r), rows, st)
             if n > 0:
+                inputs_free = torch.cuda.Event() if self.overlap_wgrad else None
                 dX = mlp_hip.backward_rows(Xr, hs, Ws, prec, self._packed, dlogit, st,
-                                           self.mlp_grads, M_dev=m_dev, bits=mbits)
+                                           self.mlp_grads, M_dev=m_dev, bits=mbits,
+                                           wstream=self.wside if self.overlap_wgrad else None,
+                                           inputs_free=inputs_free)
+                self._inputs_free = inputs_free
             else:
                 # no ray: every rgbnet gradient is zero
                 self.mlp_gflat.zero_()
@@ -381,6 +420,12 @@ class FusedFineTrainer:
         call("vx_sum_f32", ptr(self.loss_r), n, 1.0 / (3.0 * n), ptr(loss), st)
         if layer_ev:
             self.layer_events += [(e0, e1, k, m) for e0, e1, k in layer_ev]
+        # the rgbnet Adam pass goes on the third stream behind the weight gradients, which
+        # may still be running there; it also reads the output layer's gradient, reduced
+        # on the main stream
+        wgrad_side = self.overlap_wgrad and n > 0 and not profile
+        if self.overlap_wgrad and not wgrad_side:
+            main.wait_stream(self.wside)
         if profile:
             # snapshot before any optimiser pass clears a gradient
             k0_grad_snapshot = self.k0_grad.clone()
@@ -390,9 +435,18 @@ class FusedFineTrainer:
             # pages phase 1 swept this step: touched before it and not flagged (phase 2
             # of this step has not run yet, so touched is still the state phase 1 read)
             self.last_swept_pages = int(((self._touched != 0) & ~flags).sum())
-        # the rgbnet gradients are final: its Adam pass (small) goes before the SDF backward,
-        # ahead of the k0 sweep on the second stream
-        self._adam("mlp", self.mlp_flat, self.mlp_gflat)
+        if wgrad_side:
+            mlp_ready = torch.cuda.Event()
+            mlp_ready.record(main)
+            with torch.cuda.stream(self.wside):
+                self.wside.wait_event(mlp_ready)
+                self._adam("mlp", self.mlp_flat, self.mlp_gflat)
+                self._mlp_done = torch.cuda.Event()
+                self._mlp_done.record(self.wside)
+        else:
+            # the rgbnet gradients are final: its Adam pass (small) goes before the SDF
+            # backward, ahead of the k0 sweep on the second stream
+            self._adam("mlp", self.mlp_flat, self.mlp_gflat)
         if self.overlap_adam:
             # The flagged pages, once the feature scatter has finished: the sweep
             # (HBM-bound) runs beside the SDF backward (scattered atomics) and the SDF Adam
@@ -505,7 +559,7 @@ class FusedFineTrainer:
         groups, state = osd["param_groups"], osd["state"]
         if len(groups) != 3:
             raise ValueError("optimizer state: expected the groups sdf, k0 and rgbnet")
-        self.k0  # waits for a pending k0 update
+        self.wait_pending()
         for name, g in zip(("sdf", "k0"), groups[:2]):
             st = state[g["params"][0]]
             self.adam[name][0].copy_(st["exp_avg"])

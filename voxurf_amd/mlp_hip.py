@@ -118,28 +118,49 @@ def forward_rows(X, Ws, bs, prec, packed, st, M_dev=None, timing=None, bits=None
     return logits, hs
 
 
-def backward_rows(X, hs, Ws, prec, packed, g, st, grads, M_dev=None, bits=None):
+def backward_rows(X, hs, Ws, prec, packed, g, st, grads, M_dev=None, bits=None,
+                  wstream=None, inputs_free=None):
     """Gradients of every weight and bias into the tensors of `grads` ([W0, b0, W1, ...],
     overwritten) and the gradient of X [M, 32*kt0] (returned, float32); M_dev and bits
-    as filled by forward_rows."""
+    as filled by forward_rows.
+
+    wstream: optional torch stream for the hidden layers' weight gradients.  The data
+    gradients then run back to back on `st` (torch's current stream) and the weight
+    gradients are queued on wstream behind them, so the MFMA-bound weight-gradient
+    passes can run beside whatever the caller queues next on `st`; the hidden layers'
+    entries of `grads` are final once wstream reaches the point this returns at.  The
+    first of them is the input layer's, and the row count is copied on wstream before
+    it: X and M_dev may be rewritten once wstream has passed that pass, which
+    `inputs_free` (an optional torch.cuda.Event) records.  The activations passed in and
+    the tensors this function allocates are marked as used on wstream."""
     M = X.shape[0]
     dev = X.device
     nlin = len(Ws)
     part = torch.empty(lib().vx_mlp_wgrad_part_elems(M), dtype=torch.float32, device=dev)
     partb = torch.empty(lib().vx_mlp_wgrad_partb_elems(M), dtype=torch.float32, device=dev)
+    if wstream is None:
+        part_last, partb_last = part, partb
+    else:
+        # the output layer's partial sums on `st`, beside the weight-gradient passes
+        part_last = torch.empty(lib().vx_mlp_wgrad_part_elems(M), dtype=torch.float32, device=dev)
+        partb_last = torch.empty(lib().vx_mlp_wgrad_partb_elems(M), dtype=torch.float32, device=dev)
     # output layer (K = 3) and the gradient of the last hidden activations
     dH = torch.empty((M, WIDTH), dtype=torch.float32, device=dev)
     call("vx_mlp_last_bwd", prec, ptr(g), ptr(hs[-1]), WIDTH, M, ptr(Ws[-1]), ptr(dH),
-         WIDTH, ptr(part), ptr(partb), ptr(grads[2 * (nlin - 1)]), ptr(grads[2 * nlin - 1]),
-         ptr(M_dev), st)
+         WIDTH, ptr(part_last), ptr(partb_last), ptr(grads[2 * (nlin - 1)]),
+         ptr(grads[2 * nlin - 1]), ptr(M_dev), st)
     dX = None
+    wjobs = []
     for li in range(nlin - 2, -1, -1):
         h_prev = hs[li - 1] if li > 0 else X
         kt = 6 if li > 0 else packed.kt0
         # weight gradient of layer li: dY = dH (its output grad), X = its input
-        call("vx_mlp_wgrad", prec, kt, Ws[li].shape[1], ptr(dH), WIDTH, ptr(h_prev),
-             h_prev.shape[1], M, ptr(part), ptr(partb), ptr(grads[2 * li]),
-             ptr(grads[2 * li + 1]), ptr(M_dev), st)
+        if wstream is None:
+            call("vx_mlp_wgrad", prec, kt, Ws[li].shape[1], ptr(dH), WIDTH, ptr(h_prev),
+                 h_prev.shape[1], M, ptr(part), ptr(partb), ptr(grads[2 * li]),
+                 ptr(grads[2 * li + 1]), ptr(M_dev), st)
+        else:
+            wjobs.append((li, kt, dH, h_prev))
         # data gradient through layer li
         if li > 0:
             dprev = torch.empty((M, WIDTH), dtype=torch.float32, device=dev)
@@ -151,6 +172,23 @@ def backward_rows(X, hs, Ws, prec, packed, g, st, grads, M_dev=None, bits=None):
             dX = torch.empty((M, 32 * kt), dtype=torch.float32, device=dev)
             call("vx_mlp_layer_bwd", prec, kt, ptr(dH), WIDTH, M, ptr(packed.bwd[0]), None,
                  0, ptr(dX), 32 * kt, ptr(M_dev), None, st)
+    if wjobs:
+        ready = torch.cuda.Event()
+        ready.record(torch.cuda.current_stream(dev))
+        wstream.wait_event(ready)
+        ws = wstream.cuda_stream
+        with torch.cuda.stream(wstream):
+            Mw = M_dev.clone() if M_dev is not None else None
+        for i, (li, kt, dY, h_prev) in enumerate(sorted(wjobs, key=lambda j: j[0])):
+            call("vx_mlp_wgrad", prec, kt, Ws[li].shape[1], ptr(dY), WIDTH, ptr(h_prev),
+                 h_prev.shape[1], M, ptr(part), ptr(partb), ptr(grads[2 * li]),
+                 ptr(grads[2 * li + 1]), ptr(Mw), ws)
+            dY.record_stream(wstream)
+            h_prev.record_stream(wstream)
+            if i == 0 and inputs_free is not None:
+                inputs_free.record(wstream)
+        part.record_stream(wstream)
+        partb.record_stream(wstream)
     return dX
 
 
