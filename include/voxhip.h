@@ -470,6 +470,15 @@ int vx_mlp_wgrad(int prec, int kt, int k_real, const float* dY, int dys, const v
 int vx_mlp_last_bwd(int prec, const float* dlogit, const void* H, int hs, int64_t M,
                     const float* w3, float* dH, int dhs, float* part, float* partb, float* dW3,
                     float* db3, const int64_t* M_dev, void* stream);
+/* The two halves of vx_mlp_last_bwd, for callers that run them on different streams:
+ * the data gradient dH = (dlogit W3) * mask from the last hidden layer's mask bits as
+ * written by vx_mlp_layer_fwd (prec 2; H itself is not read), and dW3, db3 (the pass
+ * over H). */
+int vx_mlp_last_dgrad(const float* dlogit, const uint32_t* mask_bits, int64_t M, const float* w3,
+                      float* dH, int dhs, const int64_t* M_dev, void* stream);
+int vx_mlp_last_wgrad(int prec, const float* dlogit, const void* H, int hs, int64_t M,
+                      float* part, float* partb, float* dW3, float* db3, const int64_t* M_dev,
+                      void* stream);
 /* y = bfloat16(x), n elements. */
 int vx_f32_to_bf16(const float* x, int64_t n, void* y, void* stream);
 

@@ -184,3 +184,36 @@ def test_split_as_accurate_as_fp32_mfma(dev, din, M):
         e_n, e_s = rel_err(a_n, b_n), rel_err(a_s, b_s)
         print(f"grad {n}: split {e_s:.3g}, fp32 mfma {e_n:.3g}")
         assert e_s <= 8 * e_n + 1e-7, n
+
+
+@pytest.mark.parametrize("M", [5000, 37, 1])
+def test_backward_on_weight_stream_matches(dev, M):
+    """backward_rows with the weight gradients on a second stream (and the output layer's
+    data gradient from the mask bits, vx_mlp_last_dgrad / vx_mlp_last_wgrad): every
+    gradient bit-identical to the one-stream pass (the same kernels on the same data,
+    each with a fixed summation order)."""
+    torch.manual_seed(3)
+    net = RgbNet(42, 192, 4).to(dev)
+    lins = net.linears()
+    Ws = [l.weight.detach().contiguous() for l in lins]
+    bs = [l.bias.detach().contiguous() for l in lins]
+    prec = mlp_hip.PREC_F32_SPLIT
+    st = stream_of(Ws[0])
+    X = mlp_hip.padded_rows(torch.randn(M, 42, device=dev), 42, prec)
+    packed = mlp_hip.Packed(Ws, prec, st)
+    bits = []
+    _, hs = mlp_hip.forward_rows(X, Ws, bs, prec, packed, st, bits=bits)
+    g = torch.randn(M, 3, device=dev)
+    M_dev = torch.tensor([M], dtype=torch.int64, device=dev)
+    outs = []
+    for ws in (None, torch.cuda.Stream(dev)):
+        grads = [torch.full_like(t, float("nan")) for W, b in zip(Ws, bs) for t in (W, b)]
+        free = torch.cuda.Event() if ws is not None else None
+        dX = mlp_hip.backward_rows(X, hs, Ws, prec, packed, g, st, grads, M_dev=M_dev,
+                                   bits=bits, wstream=ws, inputs_free=free)
+        if ws is not None:
+            torch.cuda.current_stream(dev).wait_stream(ws)
+        outs.append([dX] + grads)
+    torch.cuda.synchronize()
+    for i, (a, b) in enumerate(zip(*outs)):
+        assert torch.equal(a, b), i

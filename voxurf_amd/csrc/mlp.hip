@@ -716,10 +716,10 @@ __global__ void __launch_bounds__(256) k_mlp_logits(const void* __restrict__ H_,
 }
 
 // Backward through the 3-output layer, one pass over H:
-//   dH [M, 192] = (dlogit W3) * [H > 0],
+//   dH [M, 192] = (dlogit W3) * [H > 0]   (DH true),
 //   part3[blk][c * 192 + f] = sum over the block's rows of dlogit[r][c] H[r][f],
 //   partb3[blk][c]          = sum over the block's rows of dlogit[r][c].
-template <bool BF>
+template <bool BF, bool DH>
 __global__ void __launch_bounds__(192) k_mlp_last_bwd(const float* __restrict__ dlogit,
                                                       const void* __restrict__ H_, int hs,
                                                       int64_t M, const int64_t* __restrict__ Md,
@@ -728,7 +728,8 @@ __global__ void __launch_bounds__(192) k_mlp_last_bwd(const float* __restrict__ 
                                                       float* __restrict__ part3,
                                                       float* __restrict__ partb3) {
   const int f = threadIdx.x;
-  const float w0 = w3[f], w1 = w3[kWidth + f], w2 = w3[2 * kWidth + f];
+  const float w0 = DH ? w3[f] : 0.f, w1 = DH ? w3[kWidth + f] : 0.f;
+  const float w2 = DH ? w3[2 * kWidth + f] : 0.f;
   M = eff_rows(M, Md);
   const int64_t rows_per_block = (M + gridDim.x - 1) / gridDim.x;
   const int64_t r_begin = blockIdx.x * rows_per_block;
@@ -739,8 +740,10 @@ __global__ void __launch_bounds__(192) k_mlp_last_bwd(const float* __restrict__ 
     const float hv = BF ? (float)reinterpret_cast<const __bf16*>(H_)[r * hs + f]
                         : reinterpret_cast<const float*>(H_)[r * hs + f];
     const float g0 = dlogit[3 * r], g1 = dlogit[3 * r + 1], g2 = dlogit[3 * r + 2];
-    const float g = g0 * w0 + g1 * w1 + g2 * w2;
-    dH[r * dhs + f] = hv > 0.f ? g : 0.f;
+    if (DH) {
+      const float g = g0 * w0 + g1 * w1 + g2 * w2;
+      dH[r * dhs + f] = hv > 0.f ? g : 0.f;
+    }
     a0 += g0 * hv;
     a1 += g1 * hv;
     a2 += g2 * hv;
@@ -756,6 +759,37 @@ __global__ void __launch_bounds__(192) k_mlp_last_bwd(const float* __restrict__ 
     partb3[3 * blockIdx.x] = c0;
     partb3[3 * blockIdx.x + 1] = c1;
     partb3[3 * blockIdx.x + 2] = c2;
+  }
+}
+
+// dH [M, 192] = (dlogit W3) * mask from the last hidden layer's ReLU mask bits (bit
+// f % 32 of word f / 32 of a row's six words is column f's).  A thread owns four columns
+// 4 q .. 4 q + 3 of a row (48 threads a row, four rows a block) and keeps their twelve
+// W3 entries in registers; blocks stride over the rows.
+__global__ void __launch_bounds__(192) k_mlp_last_dgrad(const float* __restrict__ dlogit,
+                                                        const uint32_t* __restrict__ bits,
+                                                        int64_t M, const int64_t* __restrict__ Md,
+                                                        const float* __restrict__ w3,
+                                                        float* __restrict__ dH, int dhs) {
+  constexpr int Q = kWidth / 4;
+  const int q = threadIdx.x % Q, rb = threadIdx.x / Q;
+  float w[3][4];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) w[c][e] = w3[c * kWidth + 4 * q + e];
+  M = eff_rows(M, Md);
+  const int shift = 4 * (q % 8);
+  for (int64_t r = (int64_t)blockIdx.x * 4 + rb; r < M; r += (int64_t)gridDim.x * 4) {
+    const float g0 = dlogit[3 * r], g1 = dlogit[3 * r + 1], g2 = dlogit[3 * r + 2];
+    const uint32_t word = bits[r * (kWidth / 32) + q / 8] >> shift;
+    float v[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const float g = g0 * w[0][e] + g1 * w[1][e] + g2 * w[2][e];
+      v[e] = (word >> e) & 1u ? g : 0.f;
+    }
+    *reinterpret_cast<float4*>(dH + r * dhs + 4 * q) = make_float4(v[0], v[1], v[2], v[3]);
   }
 }
 
@@ -1228,12 +1262,40 @@ int vx_mlp_last_bwd(int prec, const float* dlogit, const void* H, int hs, int64_
   VX_CHECK_ARG(M >= 0, "mlp_last_bwd: M < 0");
   hipStream_t st = (hipStream_t)stream;
   const int nblk = last_blocks(M);
-  if (bf16) k_mlp_last_bwd<true><<<nblk, kWidth, 0, st>>>(dlogit, H, hs, M, M_dev, w3, dH, dhs, part, partb);
-  else k_mlp_last_bwd<false><<<nblk, kWidth, 0, st>>>(dlogit, H, hs, M, M_dev, w3, dH, dhs, part, partb);
+  if (bf16) k_mlp_last_bwd<true, true><<<nblk, kWidth, 0, st>>>(dlogit, H, hs, M, M_dev, w3, dH, dhs, part, partb);
+  else k_mlp_last_bwd<false, true><<<nblk, kWidth, 0, st>>>(dlogit, H, hs, M, M_dev, w3, dH, dhs, part, partb);
   const RedJob jw{part, part + (int64_t)nblk * 3 * kWidth, dW3, 3 * kWidth, 3, kWidth, kWidth};
   const RedJob jb{partb, partb + (int64_t)nblk * 3, db3, 3, 1, 3, 3};
   reduce_two(jw, jb, nblk, st);
   return launch_status("mlp_last_bwd");
+}
+
+int vx_mlp_last_dgrad(const float* dlogit, const uint32_t* mask_bits, int64_t M, const float* w3,
+                      float* dH, int dhs, const int64_t* M_dev, void* stream) {
+  VX_CHECK_ARG(M >= 0, "mlp_last_dgrad: M < 0");
+  VX_CHECK_ARG(dhs >= kWidth && dhs % 4 == 0, "mlp_last_dgrad: dH stride must be >= 192, a multiple of 4");
+  if (M == 0) return VX_OK;
+  VX_CHECK_ARG(dlogit && mask_bits && w3 && dH, "mlp_last_dgrad: null pointer");
+  const int64_t cap = 16 * (int64_t)num_cus();
+  const int64_t b = cdiv(M, 4);
+  k_mlp_last_dgrad<<<(unsigned)(b < cap ? b : cap), kWidth, 0, (hipStream_t)stream>>>(
+      dlogit, mask_bits, M, M_dev, w3, dH, dhs);
+  return launch_status("mlp_last_dgrad");
+}
+
+int vx_mlp_last_wgrad(int prec, const float* dlogit, const void* H, int hs, int64_t M,
+                      float* part, float* partb, float* dW3, float* db3, const int64_t* M_dev,
+                      void* stream) {
+  VX_CHECK_ARG(prec >= 0 && prec <= 2, "mlp_last_wgrad: prec must be 0, 1 or 2");
+  VX_CHECK_ARG(M >= 0, "mlp_last_wgrad: M < 0");
+  hipStream_t st = (hipStream_t)stream;
+  const int nblk = last_blocks(M);
+  if (prec == 1) k_mlp_last_bwd<true, false><<<nblk, kWidth, 0, st>>>(dlogit, H, hs, M, M_dev, nullptr, nullptr, 0, part, partb);
+  else k_mlp_last_bwd<false, false><<<nblk, kWidth, 0, st>>>(dlogit, H, hs, M, M_dev, nullptr, nullptr, 0, part, partb);
+  const RedJob jw{part, part + (int64_t)nblk * 3 * kWidth, dW3, 3 * kWidth, 3, kWidth, kWidth};
+  const RedJob jb{partb, partb + (int64_t)nblk * 3, db3, 3, 1, 3, 3};
+  reduce_two(jw, jb, nblk, st);
+  return launch_status("mlp_last_wgrad");
 }
 
 int vx_f32_to_bf16(const float* x, int64_t n, void* y, void* stream) {

@@ -24,7 +24,7 @@ this step, from the end of step 1 -- concurrently with steps 2-7 -- and the flag
 pages after step 7, beside the SDF backward.  The SDF grid's pass is split the same way
 (phase 1 on the second stream, phase 2 after step 8 on the main stream).  The main stream waits for it only before
 the next step's gather reads k0, so it may also overlap the next forward.  rgbnet's
-hidden-layer weight gradients (MFMA-bound) and its Adam pass run on a third stream,
+weight gradients (MFMA-bound) and its Adam pass run on a third stream,
 queued once step 6's data gradients are done, beside steps 7-8 (atomic- and HBM-bound)
 and on into the next step's forward: that step's survivor scan waits only for the input
 layer's pass (the first, the one that reads the rows and the count), its weight packing

@@ -124,11 +124,12 @@ def backward_rows(X, hs, Ws, prec, packed, g, st, grads, M_dev=None, bits=None,
     overwritten) and the gradient of X [M, 32*kt0] (returned, float32); M_dev and bits
     as filled by forward_rows.
 
-    wstream: optional torch stream for the hidden layers' weight gradients.  The data
+    wstream: optional torch stream for the weight gradients (the output layer's too when
+    `bits` holds the mask bits: its data gradient then comes from them).  The data
     gradients then run back to back on `st` (torch's current stream) and the weight
     gradients are queued on wstream behind them, so the MFMA-bound weight-gradient
-    passes can run beside whatever the caller queues next on `st`; the hidden layers'
-    entries of `grads` are final once wstream reaches the point this returns at.  The
+    passes can run beside whatever the caller queues next on `st`; the entries of
+    `grads` computed there are final once wstream reaches the point this returns at.  The
     first of them is the input layer's, and the row count is copied on wstream before
     it: X and M_dev may be rewritten once wstream has passed that pass, which
     `inputs_free` (an optional torch.cuda.Event) records.  The activations passed in and
@@ -138,19 +139,27 @@ def backward_rows(X, hs, Ws, prec, packed, g, st, grads, M_dev=None, bits=None,
     nlin = len(Ws)
     part = torch.empty(lib().vx_mlp_wgrad_part_elems(M), dtype=torch.float32, device=dev)
     partb = torch.empty(lib().vx_mlp_wgrad_partb_elems(M), dtype=torch.float32, device=dev)
-    if wstream is None:
-        part_last, partb_last = part, partb
-    else:
-        # the output layer's partial sums on `st`, beside the weight-gradient passes
-        part_last = torch.empty(lib().vx_mlp_wgrad_part_elems(M), dtype=torch.float32, device=dev)
-        partb_last = torch.empty(lib().vx_mlp_wgrad_partb_elems(M), dtype=torch.float32, device=dev)
     # output layer (K = 3) and the gradient of the last hidden activations
     dH = torch.empty((M, WIDTH), dtype=torch.float32, device=dev)
-    call("vx_mlp_last_bwd", prec, ptr(g), ptr(hs[-1]), WIDTH, M, ptr(Ws[-1]), ptr(dH),
-         WIDTH, ptr(part_last), ptr(partb_last), ptr(grads[2 * (nlin - 1)]),
-         ptr(grads[2 * nlin - 1]), ptr(M_dev), st)
-    dX = None
     wjobs = []
+    if wstream is not None and bits:
+        # data gradient from the mask bits on `st`; dW3, db3 (a pass over H) on wstream
+        call("vx_mlp_last_dgrad", ptr(g), ptr(bits[-1]), M, ptr(Ws[-1]), ptr(dH), WIDTH,
+             ptr(M_dev), st)
+        wjobs.append((nlin - 1, 0, g, hs[-1]))
+    else:
+        if wstream is None:
+            part_last, partb_last = part, partb
+        else:
+            # the output layer's partial sums on `st`, beside the weight-gradient passes
+            part_last = torch.empty(lib().vx_mlp_wgrad_part_elems(M), dtype=torch.float32,
+                                    device=dev)
+            partb_last = torch.empty(lib().vx_mlp_wgrad_partb_elems(M), dtype=torch.float32,
+                                     device=dev)
+        call("vx_mlp_last_bwd", prec, ptr(g), ptr(hs[-1]), WIDTH, M, ptr(Ws[-1]), ptr(dH),
+             WIDTH, ptr(part_last), ptr(partb_last), ptr(grads[2 * (nlin - 1)]),
+             ptr(grads[2 * nlin - 1]), ptr(M_dev), st)
+    dX = None
     for li in range(nlin - 2, -1, -1):
         h_prev = hs[li - 1] if li > 0 else X
         kt = 6 if li > 0 else packed.kt0
@@ -180,9 +189,13 @@ def backward_rows(X, hs, Ws, prec, packed, g, st, grads, M_dev=None, bits=None,
         with torch.cuda.stream(wstream):
             Mw = M_dev.clone() if M_dev is not None else None
         for i, (li, kt, dY, h_prev) in enumerate(sorted(wjobs, key=lambda j: j[0])):
-            call("vx_mlp_wgrad", prec, kt, Ws[li].shape[1], ptr(dY), WIDTH, ptr(h_prev),
-                 h_prev.shape[1], M, ptr(part), ptr(partb), ptr(grads[2 * li]),
-                 ptr(grads[2 * li + 1]), ptr(Mw), ws)
+            if li == nlin - 1:
+                call("vx_mlp_last_wgrad", prec, ptr(dY), ptr(h_prev), WIDTH, M, ptr(part),
+                     ptr(partb), ptr(grads[2 * li]), ptr(grads[2 * li + 1]), ptr(Mw), ws)
+            else:
+                call("vx_mlp_wgrad", prec, kt, Ws[li].shape[1], ptr(dY), WIDTH, ptr(h_prev),
+                     h_prev.shape[1], M, ptr(part), ptr(partb), ptr(grads[2 * li]),
+                     ptr(grads[2 * li + 1]), ptr(Mw), ws)
             dY.record_stream(wstream)
             h_prev.record_stream(wstream)
             if i == 0 and inputs_free is not None:
