@@ -461,10 +461,12 @@ int vx_mlp_layer_bwd(int prec, int nto, const float* dy, int dy_stride, int64_t 
 int64_t vx_mlp_wgrad_part_elems(int64_t M);
 int64_t vx_mlp_wgrad_partb_elems(int64_t M);
 /* dW [192, k_real] = dY^T X, db [192] = column sums of dY (split over row blocks, then
- * a deterministic two-pass reduction). */
+ * a deterministic two-pass reduction).  max_blocks: 0 for one row block per CU (the
+ * fastest alone), or fewer, to leave CUs to kernels running beside it on other
+ * streams (the result does not depend on it beyond FP32 summation order). */
 int vx_mlp_wgrad(int prec, int kt, int k_real, const float* dY, int dys, const void* X, int xs,
                  int64_t M, float* part, float* partb, float* dW, float* db, const int64_t* M_dev,
-                 void* stream);
+                 int max_blocks, void* stream);
 /* Backward through the 3-output layer in one pass over H: dH = (dlogit W3) * [H > 0],
  * dW3 [3, 192] = dlogit^T H, db3 [3] = column sums of dlogit. */
 int vx_mlp_last_bwd(int prec, const float* dlogit, const void* H, int hs, int64_t M,

@@ -1038,10 +1038,12 @@ static void launch_layer_split(const void* in, int in_stride, int64_t M, const i
 
 // Row blocks of the weight-gradient kernel: one per CU (each holds two row chunks in
 // LDS), at least one chunk of rows per block.
-static int wgrad_blocks(int64_t M, int kt) {
+// One block per CU at most (each holds 96 KB of LDS), or max_blocks when that is
+// smaller and positive.
+static int wgrad_blocks(int64_t M, int kt, int max_blocks = 0) {
   (void)kt;
   int64_t b = cdiv(M, kWgradChunk);
-  const int64_t cap = num_cus();
+  const int64_t cap = max_blocks > 0 && max_blocks < num_cus() ? max_blocks : num_cus();
   if (b > cap) b = cap;
   return (int)(b < 1 ? 1 : b);
 }
@@ -1234,14 +1236,15 @@ int64_t vx_mlp_wgrad_partb_elems(int64_t M) {
 
 int vx_mlp_wgrad(int prec, int kt, int k_real, const float* dY, int dys, const void* X, int xs,
                  int64_t M, float* part, float* partb, float* dW, float* db, const int64_t* M_dev,
-                 void* stream) {
+                 int max_blocks, void* stream) {
   VX_CHECK_ARG(prec >= 0 && prec <= 2, "mlp_wgrad: prec must be 0, 1 or 2");
   const int bf16 = prec == 1;
   VX_CHECK_ARG(kt == 2 || kt == 3 || kt == 6, "mlp_wgrad: input width must be 64, 96 or 192");
   VX_CHECK_ARG(k_real >= 1 && k_real <= 32 * kt, "mlp_wgrad: k_real out of range");
   VX_CHECK_ARG(M >= 0, "mlp_wgrad: M < 0");
   hipStream_t st = (hipStream_t)stream;
-  const int nblk = wgrad_blocks(M, kt);
+  VX_CHECK_ARG(max_blocks >= 0, "mlp_wgrad: max_blocks < 0");
+  const int nblk = wgrad_blocks(M, kt, max_blocks);
   const int K = 32 * kt;
 #define L(KT, BF) k_mlp_wgrad<KT, BF><<<nblk, 64 * WgradShape<KT>::WAVES, 0, st>>>(dY, dys, X, xs, M, M_dev, part, partb)
   if (kt == 2) { if (bf16) L(2, true); else L(2, false); }

@@ -119,7 +119,7 @@ def forward_rows(X, Ws, bs, prec, packed, st, M_dev=None, timing=None, bits=None
 
 
 def backward_rows(X, hs, Ws, prec, packed, g, st, grads, M_dev=None, bits=None,
-                  wstream=None, inputs_free=None):
+                  wstream=None, inputs_free=None, wblocks=0):
     """Gradients of every weight and bias into the tensors of `grads` ([W0, b0, W1, ...],
     overwritten) and the gradient of X [M, 32*kt0] (returned, float32); M_dev and bits
     as filled by forward_rows.
@@ -133,7 +133,9 @@ def backward_rows(X, hs, Ws, prec, packed, g, st, grads, M_dev=None, bits=None,
     first of them is the input layer's, and the row count is copied on wstream before
     it: X and M_dev may be rewritten once wstream has passed that pass, which
     `inputs_free` (an optional torch.cuda.Event) records.  The activations passed in and
-    the tensors this function allocates are marked as used on wstream."""
+    the tensors this function allocates are marked as used on wstream.  wblocks: row
+    blocks of the hidden layers' weight-gradient passes there (0: one per CU; fewer
+    leave CUs to the kernels beside them)."""
     M = X.shape[0]
     dev = X.device
     nlin = len(Ws)
@@ -167,7 +169,7 @@ def backward_rows(X, hs, Ws, prec, packed, g, st, grads, M_dev=None, bits=None,
         if wstream is None:
             call("vx_mlp_wgrad", prec, kt, Ws[li].shape[1], ptr(dH), WIDTH, ptr(h_prev),
                  h_prev.shape[1], M, ptr(part), ptr(partb), ptr(grads[2 * li]),
-                 ptr(grads[2 * li + 1]), ptr(M_dev), st)
+                 ptr(grads[2 * li + 1]), ptr(M_dev), 0, st)
         else:
             wjobs.append((li, kt, dH, h_prev))
         # data gradient through layer li
@@ -195,7 +197,7 @@ def backward_rows(X, hs, Ws, prec, packed, g, st, grads, M_dev=None, bits=None,
             else:
                 call("vx_mlp_wgrad", prec, kt, Ws[li].shape[1], ptr(dY), WIDTH, ptr(h_prev),
                      h_prev.shape[1], M, ptr(part), ptr(partb), ptr(grads[2 * li]),
-                     ptr(grads[2 * li + 1]), ptr(Mw), ws)
+                     ptr(grads[2 * li + 1]), ptr(Mw), wblocks, ws)
             dY.record_stream(wstream)
             h_prev.record_stream(wstream)
             if i == 0 and inputs_free is not None:
