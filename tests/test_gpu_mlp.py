@@ -217,3 +217,35 @@ def test_backward_on_weight_stream_matches(dev, M):
     torch.cuda.synchronize()
     for i, (a, b) in enumerate(zip(*outs)):
         assert torch.equal(a, b), i
+
+
+def test_weight_gradient_block_count(dev):
+    """vx_mlp_wgrad's max_blocks changes only how rows are split between row blocks
+    (the FP32 summation order): weight and bias gradients with 64 and 200 blocks within
+    1e-5 relative (of the largest entry) of those with one block per CU."""
+    torch.manual_seed(4)
+    M = 30000
+    net = RgbNet(42, 192, 4).to(dev)
+    lins = net.linears()
+    Ws = [l.weight.detach().contiguous() for l in lins]
+    bs = [l.bias.detach().contiguous() for l in lins]
+    prec = mlp_hip.PREC_F32_SPLIT
+    st = stream_of(Ws[0])
+    X = mlp_hip.padded_rows(torch.randn(M, 42, device=dev), 42, prec)
+    packed = mlp_hip.Packed(Ws, prec, st)
+    bits = []
+    _, hs = mlp_hip.forward_rows(X, Ws, bs, prec, packed, st, bits=bits)
+    g = torch.randn(M, 3, device=dev)
+    outs = []
+    for wb in (0, 64, 200):
+        ws = torch.cuda.Stream(dev)
+        grads = [torch.full_like(t, float("nan")) for W, b in zip(Ws, bs) for t in (W, b)]
+        mlp_hip.backward_rows(X, hs, Ws, prec, packed, g, st, grads, bits=bits, wstream=ws,
+                              wblocks=wb)
+        torch.cuda.current_stream(dev).wait_stream(ws)
+        outs.append(grads)
+    torch.cuda.synchronize()
+    for other in outs[1:]:
+        for a, b in zip(outs[0], other):
+            assert torch.isfinite(b).all()
+            torch.testing.assert_close(b, a, rtol=1e-5, atol=1e-5 * float(a.abs().max()))

@@ -266,14 +266,29 @@ class FusedFineTrainer:
              int(self.count_all), ptr(self.overflow),
              self.k0_geom.data_ptr() if pages is not None else None, ptr(pages),
              ptr(self._sdf_pages) if sdf_step is not None else None, st)
+        # the previous step's input-layer weight gradient reads the rows (X) the gather
+        # rewrites, after a copy of the survivor count (off) the scan rewrites
+        if self._inputs_free is not None:
+            main.wait_event(self._inputs_free)
+            self._inputs_free = None
+        call("vx_inclusive_scan_i64", ptr(self.cnt), ptr(self.off), n, ptr(self.ws), st)
+        if n > 0:
+            # the survivor count, on its way to the host while the rest is queued
+            self._m_host.copy_(self.off[n - 1:n], non_blocking=True)
+            self._m_ready.record(main)
         if pages is not None and self.overlap_adam:
             # The forward has flagged every k0 page the feature scatter can write; the
             # other pages have an all-zero gradient this step, so their Adam update (most
             # of the 256^3 x 12 sweep, HBM-bound) starts now on the second stream and runs
             # beside the gather and the MFMA-bound rgbnet passes.  The gather reads k0
             # only at survivor corners, i.e. in flagged pages, which this pass leaves alone.
-            flags_ready = torch.cuda.Event()
-            flags_ready.record(main)
+            # (queued after the survivor scan: the side stream waits for the event the
+            # count's copy records, one event record on the main stream instead of two)
+            if n > 0:
+                flags_ready = self._m_ready
+            else:
+                flags_ready = torch.cuda.Event()
+                flags_ready.record(main)
             with torch.cuda.stream(self.side):
                 self.side.wait_event(flags_ready)
                 self._adam("k0", self._k0, self.k0_grad, pages=pages, phase=1, step=k0_step,
@@ -286,16 +301,6 @@ class FusedFineTrainer:
                                step=sdf_step, touched=self._sdf_touched)
                     self._sdf_phase1 = torch.cuda.Event()
                     self._sdf_phase1.record(self.side)
-        # the previous step's input-layer weight gradient reads the rows (X) the gather
-        # rewrites, after a copy of the survivor count (off) the scan rewrites
-        if self._inputs_free is not None:
-            main.wait_event(self._inputs_free)
-            self._inputs_free = None
-        call("vx_inclusive_scan_i64", ptr(self.cnt), ptr(self.off), n, ptr(self.ws), st)
-        if n > 0:
-            # the survivor count, on its way to the host while the rest is queued
-            self._m_host.copy_(self.off[n - 1:n], non_blocking=True)
-            self._m_ready.record(main)
         # the previous step's k0 update must have finished before k0 is read
         if self._k0_done is not None:
             main.wait_event(self._k0_done)
@@ -426,8 +431,7 @@ class FusedFineTrainer:
         if layer_ev:
             self.layer_events += [(e0, e1, k, m) for e0, e1, k in layer_ev]
         # the rgbnet Adam pass goes on the third stream behind the weight gradients, which
-        # may still be running there; it also reads the output layer's gradient, reduced
-        # on the main stream
+        # may still be running there
         wgrad_side = self.overlap_wgrad and n > 0 and not profile
         if self.overlap_wgrad and not wgrad_side:
             main.wait_stream(self.wside)
@@ -441,10 +445,10 @@ class FusedFineTrainer:
             # of this step has not run yet, so touched is still the state phase 1 read)
             self.last_swept_pages = int(((self._touched != 0) & ~flags).sum())
         if wgrad_side:
-            mlp_ready = torch.cuda.Event()
-            mlp_ready.record(main)
+            # every gradient it reads was written on the third stream or before the
+            # event that stream waited for (the end of the data gradients), and the main
+            # stream reads no rgbnet weight after that event this step
             with torch.cuda.stream(self.wside):
-                self.wside.wait_event(mlp_ready)
                 self._adam("mlp", self.mlp_flat, self.mlp_gflat)
                 self._mlp_done = torch.cuda.Event()
                 self._mlp_done.record(self.wside)
