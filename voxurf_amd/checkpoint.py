@@ -101,6 +101,8 @@ def maskcache_kwargs(mc):
 
 def _optimizer_state(trainer):
     """torch.optim-style state of the trainer's Adam moments: groups sdf, k0, rgbnet."""
+    if hasattr(trainer, "wait_pending"):
+        trainer.wait_pending()  # the k0 and rgbnet updates may still run on other streams
     state, groups = {}, []
     m, v, step, lr = trainer.adam["sdf"]
     state[0] = {"step": step, "exp_avg": m.detach().cpu(), "exp_avg_sq": v.detach().cpu()}

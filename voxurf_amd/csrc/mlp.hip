@@ -1036,10 +1036,9 @@ static void launch_layer_split(const void* in, int in_stride, int64_t M, const i
       out_stride, bits);
 }
 
-// Row blocks of the weight-gradient kernel: one per CU (each holds two row chunks in
-// LDS), at least one chunk of rows per block.
-// One block per CU at most (each holds 96 KB of LDS), or max_blocks when that is
-// smaller and positive.
+// Row blocks of the weight-gradient kernel: at least one chunk of rows per block, at
+// most one block per CU (each holds two row chunks, up to 96 KB of LDS), or max_blocks
+// when that is smaller and positive.
 static int wgrad_blocks(int64_t M, int kt, int max_blocks = 0) {
   (void)kt;
   int64_t b = cdiv(M, kWgradChunk);
